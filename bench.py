@@ -1,0 +1,156 @@
+"""Benchmark: MLS-MPM substeps of the 256^3 channel workload on one MI355X.
+
+Prints one JSON line.  A "step" is one MLS-MPM substep (P2G, grid update,
+G2P, and the Morton re-sort every RESORT_EVERY substeps) over all particles,
+with the state resident in HBM.  `value` is particle-substeps per second over
+the K timed substeps (one engine call, so the fused G2P+P2G kernel runs
+K-1 times).  `roofline` describes the dominant kernel (fused G2P+P2G),
+timed with HIP events on the engine's stream inside the timed region.
+`cpu_baseline` times the oracle's OpenMP step on the host cores on a
+bounded sample (a few substeps of the same scene).
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "particle-substeps/s and % HBM roofline at 1 GPU; learned-4xDT vs plain sim-s/s at RMS err"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per launch of the fused G2P+P2G kernel: per particle it
+# reads position (12 B) and F (36 B) and writes both back (48 B); v and C
+# stay in registers.  Per active grid node it reads the velocity (12 B) and
+# the SDF (4 B) and writes mass and momentum (16 B).
+BYTES_PER_PARTICLE = 96
+BYTES_PER_ACTIVE_NODE = 32
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000, help="timed substeps")
+    ap.add_argument("--warmup", type=int, default=80, help="untimed substeps")
+    ap.add_argument("--scene", default="channel256")
+    ap.add_argument("--resort-every", type=int, default=None)
+    ap.add_argument("--chunk", type=int, default=None)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timing", action="store_true", help="no per-kernel events")
+    ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes per launch")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: the engine is single-GPU (multi-GPU slab "
+              "decomposition is a design, not part of this build); use --gpus 1", file=sys.stderr)
+        return 2
+
+    mpm = importlib.import_module("water-sim-cuda_amd")
+    t0 = time.time()
+    sc = mpm.scenes.SCENES[args.scene]()
+    cfg = sc["cfg"]
+    n = sc["particles"].shape[1]
+    resort = args.resort_every or sc["resort_every"]
+    log(f"scene {sc['name']}: n={n} grid={cfg.SIZE_X}^3 (host gen {time.time() - t0:.1f}s)")
+
+    opts = {"RESORT_EVERY": resort, "TIMING": 0 if args.no_timing else 1}
+    if args.chunk:
+        opts["CHUNK"] = args.chunk
+    eng = mpm.Engine(cfg, **opts)
+    eng.init(n, sc["particles"])
+    t0 = time.time()
+    if sc.get("sdf") is not None:
+        eng.set_sdf(sc["sdf"])
+    else:
+        eng.initSDF(sc["triangles"])
+    log(f"sdf ready ({time.time() - t0:.2f}s, {0 if sc.get('triangles') is None else len(sc['triangles'])} tris)")
+
+    eng.steps(args.warmup)
+    eng.synchronize()
+    eng.stage_times()  # drop warm-up events
+    log(f"warmup {args.warmup} substeps done")
+
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.steps(args.steps)
+    eng.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    st = eng.stage_times()
+    ctr = eng.counters()
+    grid = eng.getGrid()
+    active = int((grid[0] > 1e-9).sum())
+    state = eng.getParticles()
+    finite = bool(np.isfinite(state[0:6]).all())
+    log(f"timed {args.steps} substeps in {elapsed:.3f}s; stages {st}; counters {ctr}; active nodes {active}")
+
+    value = n * args.steps / elapsed
+    roof = None
+    ms, cnt = st["g2p2g"]
+    if cnt > 0:
+        avg_s = ms / cnt / 1e3
+        bytes_launch = BYTES_PER_PARTICLE * n + BYTES_PER_ACTIVE_NODE * active
+        achieved = bytes_launch / avg_s / 1e9
+        traffic = None
+        if args.traffic and Path(args.traffic).exists():
+            traffic = json.loads(Path(args.traffic).read_text()).get("hbm_bytes_per_launch")
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "k_transfer<G2P2G>", "avg_launch_ms": round(ms / cnt, 5),
+                "algorithmic_bytes_per_launch": bytes_launch}
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+
+        nthreads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
+        sdf = eng.get_sdf()
+        st0 = oracle.init_state(cfg, sc["particles"], np.float32)
+        t_one, _ = oracle.time_steps(cfg, st0, sdf, 1, nthreads)
+        k = max(1, min(20, int(args.cpu_seconds / max(t_one, 1e-3))))
+        t_k, _ = oracle.time_steps(cfg, st0, sdf, k, nthreads)
+        cpu = {"value": round(n * k / t_k, 1), "unit": "particle-substeps/s", "cores": nthreads, "kind": "port",
+               "sample": f"{k} substeps of the full {sc['name']} scene (n={n}) with the OpenMP oracle, "
+                         f"{t_k:.1f}s wall"}
+        log(f"cpu baseline: {cpu}")
+    eng.close()
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": sc["name"], "grid": [cfg.SIZE_X, cfg.SIZE_Y, cfg.SIZE_Z], "particles": n,
+                   "dt": cfg.DT, "resort_every": resort, "description": sc["description"],
+                   "state_finite": finite},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "stage_ms_per_launch": {k: (round(v[0] / v[1], 5) if v[1] else None) for k, v in st.items()},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

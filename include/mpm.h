@@ -1,0 +1,217 @@
+/*
+ * mpm.h -- C ABI of the MI355X MLS-MPM engine (libmpm.so).
+ *
+ * Drop-in boundary for the reference's step path: the C++ class `Engine`
+ * (engine.h:14-43, engine.cu:45-114) and the config plumbing around it
+ * (SimConfig common.cuh:65-85, uploadConfigToGPU config.cu:5-8,
+ * GameConfigData game_configdata.h:15-32, Simulation::initDeviceParams
+ * Simulation.cpp:56-77).  Each entry point names the reference interface it
+ * replaces.
+ *
+ * Conventions
+ *  - Every function returns MPM_OK (0) or a negative MPM_E* code; the message
+ *    of the last failure on the calling thread is in mpm_last_error().  The
+ *    reference calls exit(1) instead (engine.cu:21-28, game_configdata.cpp:56-90).
+ *  - Host pointers are plain host memory owned by the caller; the library
+ *    copies from and to them and never frees them.  (The reference's
+ *    Engine::init takes ownership of its buffer, engine.cu:38.)
+ *  - Particle buffers use the reference's 26-attribute SoA layout
+ *    (ParticleAttr, common.cuh:12-30): element (attr, i) at attr*n + i, attrs
+ *    pos[3], vel[3], C[9] row-major, F[9] row-major, VOL, MASS.
+ *  - One engine handle per host thread; a handle's calls are synchronous with
+ *    respect to the host unless noted.  Config lives in the handle (no
+ *    process-global state), so several engines may coexist.
+ */
+#ifndef MPM_H
+#define MPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_OK 0
+#define MPM_EINVAL (-1)   /* bad argument */
+#define MPM_EHIP (-2)     /* HIP runtime error */
+#define MPM_ESTATE (-3)   /* call order violated (e.g. step before init) */
+#define MPM_EIO (-4)      /* file or parse error */
+#define MPM_ENOMEM (-5)   /* allocation failed */
+#define MPM_ELIMIT (-6)   /* size beyond a documented maximum */
+
+#define MPM_PARTICLE_ATTRS 26
+
+/* SimConfig, common.cuh:65-79 -- same fields, same order, same types.
+ * GRID_SIZE = 16*G bytes, GRID_NUMBER = G = SIZE_X*SIZE_Y*SIZE_Z,
+ * GRID_BLOCKS = ceil(G/256) are derived (Simulation.cpp:70-72). */
+typedef struct mpm_sim_config {
+    uint32_t SIZE_X, SIZE_Y, SIZE_Z;
+    uint32_t PADDING;
+    float GRAVITY;
+    float DT;
+    float GAMMA;
+    float COMPRESSION;
+    float RESOLUTION;
+    uint32_t SUBSTEPS;
+    uint32_t SDF_RESOLUTION;
+    uint32_t GRID_SIZE;
+    uint32_t GRID_NUMBER;
+    uint32_t GRID_BLOCKS;
+} mpm_sim_config;
+
+/* Triangle, common.cuh:123-126: three float3 vertices, 36 bytes. */
+typedef struct mpm_triangle {
+    float v0[3], v1[3], v2[3];
+} mpm_triangle;
+
+/* WATER= / OBJECT= entry, ConfigObject game_configdata.h:6-10. */
+typedef struct mpm_config_object {
+    char model_path[256];
+    float scale;
+    float x, y, z;
+} mpm_config_object;
+
+typedef struct mpm_engine mpm_engine;
+
+/* Last error message of the calling thread ("" if none). */
+const char* mpm_last_error(void);
+
+/* Library version (major*10000 + minor*100 + patch). */
+int mpm_version(void);
+
+/* ---- configuration (GameConfigData + initDeviceParams) ---- */
+
+/* Fills GRID_SIZE, GRID_NUMBER, GRID_BLOCKS from SIZE_* (Simulation.cpp:70-72). */
+int mpm_config_derive(mpm_sim_config* cfg);
+
+/* GameConfigData::setConfigDataFromFile (game_configdata.cpp:22-54) followed
+ * by Simulation::initDeviceParams (Simulation.cpp:56-77): KEY=VALUE lines,
+ * spaces removed, WATER=/OBJECT= as path;scale;x;y;z.  Every SimConfig key
+ * that the reference reads with getInt/getFloat is required (missing key ->
+ * MPM_EIO with the key named, where the reference exits).  `waters`/`objects`
+ * may be NULL; *n_waters/*n_objects receive the entry counts, at most
+ * `cap` entries are written to each array. */
+int mpm_config_parse_file(const char* path, mpm_sim_config* cfg, mpm_config_object* waters,
+                          int* n_waters, mpm_config_object* objects, int* n_objects, int cap);
+/* Same grammar from an in-memory string. */
+int mpm_config_parse_string(const char* text, mpm_sim_config* cfg, mpm_config_object* waters,
+                            int* n_waters, mpm_config_object* objects, int* n_objects, int cap);
+/* Extra engine keys in the same file (RESORT_EVERY=...), returned as int
+ * values; returns MPM_EIO if `key` is absent. */
+int mpm_config_get_int(const char* text, const char* key, int64_t* value);
+
+/* ---- engine lifetime (Engine ctor/dtor, engine.cu:36-43) ---- */
+
+/* Creates an engine on the current HIP device with its own stream.  The
+ * config is copied into the handle (replaces uploadConfigToGPU, config.cu:5-8,
+ * and the __constant__ d_config). */
+int mpm_engine_create(const mpm_sim_config* cfg, mpm_engine** out);
+int mpm_engine_destroy(mpm_engine* h);
+
+/* Engine options (no reference counterpart; defaults in brackets):
+ *   "RESORT_EVERY"  Morton re-sort period in substeps [1 = every substep, as
+ *                   engine.cu:57].  The sort changes only the order of the
+ *                   float atomics, not the exact-arithmetic result.
+ *   "CHUNK"         maximum particles per transfer workgroup [512]
+ *   "TIMING"        1 = record per-stage HIP events [0]            */
+int mpm_engine_set_option(mpm_engine* h, const char* key, int64_t value);
+int mpm_engine_get_option(mpm_engine* h, const char* key, int64_t* value);
+
+/* ---- particles and collider ---- */
+
+/* Engine::init + initParticles + initGrid (engine.cu:45-52, :72-90) and
+ * initFMatrices (kernels.cu:349-358): copies n particles from the 26*n host
+ * buffer, sets F's diagonal to 1, V = RESOLUTION^3, m = 1, zeroes the grid and
+ * performs the initial Morton sort.  n = 0 is allowed (steps are no-ops). */
+int mpm_engine_init_particles(mpm_engine* h, int n, const float* soa26);
+
+/* Engine::initSDF + makeSDF (engine.cu:105-114, kernels.cu:282-300): builds the
+ * node SDF from nt triangles (nt = 0 gives +1e15 everywhere, as the reference). */
+int mpm_engine_init_sdf_from_triangles(mpm_engine* h, const mpm_triangle* tris, int nt);
+
+/* Sets the node SDF directly from G = SIZE_X*SIZE_Y*SIZE_Z floats (index
+ * z*X*Y + y*X + x, getGridIdx common.cuh:131-134).  For analytic colliders. */
+int mpm_engine_set_sdf(mpm_engine* h, const float* sdf);
+int mpm_engine_get_sdf(mpm_engine* h, float* sdf_out);
+
+/* ---- stepping (Engine::step, engine.cu:54-70) ---- */
+
+/* One substep: sort (per RESORT_EVERY), clear, P2G, grid update, G2P. */
+int mpm_engine_step(mpm_engine* h);
+/* k substeps back to back (Simulation::run's inner loop, Simulation.cpp:46-49). */
+int mpm_engine_steps(mpm_engine* h, int k);
+/* Blocks until the engine's stream is idle. */
+int mpm_engine_synchronize(mpm_engine* h);
+
+/* ---- read-back ---- */
+
+/* Engine::getNumber (engine.h:31). */
+int mpm_engine_get_number(mpm_engine* h);
+/* Engine::getBuffer / getParticles (engine.h:22, :32): the full state in the
+ * caller's original particle order, 26*n floats. */
+int mpm_engine_get_particles(mpm_engine* h, float* soa26_out);
+/* Engine::getPositions / changeFormat (engine.h:24, kernels.cu:341-347):
+ * positions as x,y,z triples in the caller's order, 3*n floats. */
+int mpm_engine_get_positions(mpm_engine* h, float* xyz_out);
+/* Engine::sortParticles (engine.cu:92-102): Morton keys of the current
+ * positions (setKeys, kernels.cu:361-371) and the stable ascending key sort.
+ * keys_out receives the sorted keys, perm_out the caller indices in sorted
+ * order (the reference's d_keys and d_values after sort_by_key). */
+int mpm_engine_get_sort(mpm_engine* h, int32_t* keys_out, int32_t* perm_out);
+/* Engine::getGrid (engine.h:23): the grid after the last grid update, 4*G
+ * floats as planes mass, v_x, v_y, v_z (the reference's momentum arrays hold
+ * velocity after gridUpdate, kernels.cu:193). */
+int mpm_engine_get_grid(mpm_engine* h, float* grid4_out);
+
+/* Per-stage device time accumulated while TIMING=1, in milliseconds, and the
+ * number of launches: out[2*s] = ms, out[2*s+1] = count for stage s in
+ * 0 = P2G, 1 = G2P, 2 = fused G2P+P2G, 3 = grid update, 4 = re-sort.
+ * Resets the accumulators. */
+int mpm_engine_get_stage_times(mpm_engine* h, double* out, int nstages);
+
+/* Diagnostic counters of the last substep: out[0] = chunks, out[1] =
+ * particles whose stencil left their workgroup's LDS tile (global-atomic
+ * path), out[2] = active 4^3 grid blocks. */
+int mpm_engine_get_counters(mpm_engine* h, int64_t* out, int n);
+
+/* ---- stage access for parity tests (no reference counterpart) ---- */
+
+/* P2G of the current state alone (sort + clear + p2GTransferScatter,
+ * kernels.cu:6-142) into grid4_out (planes mass, mom_x, mom_y, mom_z).  The
+ * engine's state is unchanged. */
+int mpm_engine_debug_p2g(mpm_engine* h, float* grid4_out);
+/* gridUpdate (kernels.cu:144-196) of an arbitrary grid: grid4_in holds
+ * mass/momentum planes, grid4_out receives mass/velocity planes.  The
+ * engine's particle state is unchanged. */
+int mpm_engine_debug_grid_update(mpm_engine* h, const float* grid4_in, float* grid4_out);
+/* G2P (kernels.cu:199-280) of the current particles against the given
+ * mass/velocity grid; updates the particle state in place. */
+int mpm_engine_debug_g2p(mpm_engine* h, const float* grid4_in);
+
+/* ---- scene generators (build components; no reference counterpart) ---- */
+
+/* Jittered particles, 8 per cell (one per 2x2x2 sub-cell, uniform jitter from
+ * SplitMix64(seed)).  shape 0 = box [lo, hi) in cells, shape 1 = sphere
+ * centre lo[0..2], radius hi[0].  Appends to `soa26_out` laid out with stride
+ * `cap` (attr*cap + i) starting at index *count; with soa26_out == NULL only
+ * counts.  Positions only; all other attributes 0 except VOL = MASS = 1
+ * (VoxelPipeline::process, VoxelPipeline.cpp:9-30). */
+int mpm_scene_fill(int shape, const float* lo, const float* hi, uint64_t seed, float* soa26_out,
+                   int64_t cap, int64_t* count);
+/* UV sphere mesh: `slices` around, `stacks` pole to pole, 2*slices*(stacks-1)
+ * triangles.  tris_out may be NULL (count only). */
+int mpm_mesh_uv_sphere(const float* centre, float radius, int slices, int stacks, mpm_triangle* tris_out,
+                       int* ntris);
+/* Closed U-shaped channel along x: outer box lo..hi, wall thickness `wall`,
+ * floor thickness `wall`, faces split into squares of about `seg` cells. */
+int mpm_mesh_u_channel(const float* lo, const float* hi, float wall, float seg, mpm_triangle* tris_out,
+                       int* ntris);
+/* Axis-aligned box mesh lo..hi (12 triangles, outward winding). */
+int mpm_mesh_box(const float* lo, const float* hi, mpm_triangle* tris_out, int* ntris);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPM_H */

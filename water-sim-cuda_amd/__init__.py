@@ -1,0 +1,403 @@
+"""Python host mirror of the engine's C ABI (libmpm.so, include/mpm.h).
+
+Mirrors the reference's Engine interface (engine.h:14-43) with the same
+method names -- ``init``, ``step``, ``initSDF``, ``getNumber``,
+``getParticles``, ``getPositions``, ``getGrid``, ``sortParticles`` -- over
+ctypes, with numpy arrays at the boundary.  The compute path is the HIP
+library only: importing this package never falls back to a CPU
+implementation, and constructing an Engine without a GPU or without the
+built library raises.
+
+The directory name contains hyphens; import it with
+``importlib.import_module("water-sim-cuda_amd")`` after putting the
+repository root on ``sys.path``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libmpm.so"
+
+PARTICLE_ATTRS = 26
+ATTR_NAMES = (
+    ["pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z"]
+    + [f"C{r}{c}" for r in range(3) for c in range(3)]
+    + [f"F{r}{c}" for r in range(3) for c in range(3)]
+    + ["vol", "mass"]
+)
+
+MPM_OK = 0
+
+
+class MPMError(RuntimeError):
+    pass
+
+
+class SimConfig(ctypes.Structure):
+    """SimConfig, common.cuh:65-79 (same fields and order)."""
+
+    _fields_ = [
+        ("SIZE_X", ctypes.c_uint32),
+        ("SIZE_Y", ctypes.c_uint32),
+        ("SIZE_Z", ctypes.c_uint32),
+        ("PADDING", ctypes.c_uint32),
+        ("GRAVITY", ctypes.c_float),
+        ("DT", ctypes.c_float),
+        ("GAMMA", ctypes.c_float),
+        ("COMPRESSION", ctypes.c_float),
+        ("RESOLUTION", ctypes.c_float),
+        ("SUBSTEPS", ctypes.c_uint32),
+        ("SDF_RESOLUTION", ctypes.c_uint32),
+        ("GRID_SIZE", ctypes.c_uint32),
+        ("GRID_NUMBER", ctypes.c_uint32),
+        ("GRID_BLOCKS", ctypes.c_uint32),
+    ]
+
+    def grid_shape(self) -> tuple[int, int, int]:
+        """(SIZE_Z, SIZE_Y, SIZE_X): node arrays reshape to this (x fastest)."""
+        return (int(self.SIZE_Z), int(self.SIZE_Y), int(self.SIZE_X))
+
+    @property
+    def nodes(self) -> int:
+        return int(self.SIZE_X) * int(self.SIZE_Y) * int(self.SIZE_Z)
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ConfigObject(ctypes.Structure):
+    _fields_ = [
+        ("model_path", ctypes.c_char * 256),
+        ("scale", ctypes.c_float),
+        ("x", ctypes.c_float),
+        ("y", ctypes.c_float),
+        ("z", ctypes.c_float),
+    ]
+
+
+_lib = None
+
+
+def library_path() -> Path:
+    return LIB_PATH
+
+
+def load_library() -> ctypes.CDLL:
+    """Loads libmpm.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise MPMError(f"{LIB_PATH} is missing: build it first (python water-sim-cuda_amd/build.py)")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    c_float_p = ctypes.POINTER(ctypes.c_float)
+    c_int_p = ctypes.POINTER(ctypes.c_int)
+    c_i32_p = ctypes.POINTER(ctypes.c_int32)
+    c_i64_p = ctypes.POINTER(ctypes.c_int64)
+    c_dbl_p = ctypes.POINTER(ctypes.c_double)
+    eng = ctypes.c_void_p
+    sigs = {
+        "mpm_last_error": (ctypes.c_char_p, []),
+        "mpm_version": (ctypes.c_int, []),
+        "mpm_config_derive": (ctypes.c_int, [ctypes.POINTER(SimConfig)]),
+        "mpm_config_parse_file": (
+            ctypes.c_int,
+            [ctypes.c_char_p, ctypes.POINTER(SimConfig), ctypes.POINTER(ConfigObject), c_int_p,
+             ctypes.POINTER(ConfigObject), c_int_p, ctypes.c_int],
+        ),
+        "mpm_config_parse_string": (
+            ctypes.c_int,
+            [ctypes.c_char_p, ctypes.POINTER(SimConfig), ctypes.POINTER(ConfigObject), c_int_p,
+             ctypes.POINTER(ConfigObject), c_int_p, ctypes.c_int],
+        ),
+        "mpm_config_get_int": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, c_i64_p]),
+        "mpm_engine_create": (ctypes.c_int, [ctypes.POINTER(SimConfig), ctypes.POINTER(ctypes.c_void_p)]),
+        "mpm_engine_destroy": (ctypes.c_int, [eng]),
+        "mpm_engine_set_option": (ctypes.c_int, [eng, ctypes.c_char_p, ctypes.c_int64]),
+        "mpm_engine_get_option": (ctypes.c_int, [eng, ctypes.c_char_p, c_i64_p]),
+        "mpm_engine_init_particles": (ctypes.c_int, [eng, ctypes.c_int, c_float_p]),
+        "mpm_engine_init_sdf_from_triangles": (ctypes.c_int, [eng, c_float_p, ctypes.c_int]),
+        "mpm_engine_set_sdf": (ctypes.c_int, [eng, c_float_p]),
+        "mpm_engine_get_sdf": (ctypes.c_int, [eng, c_float_p]),
+        "mpm_engine_step": (ctypes.c_int, [eng]),
+        "mpm_engine_steps": (ctypes.c_int, [eng, ctypes.c_int]),
+        "mpm_engine_synchronize": (ctypes.c_int, [eng]),
+        "mpm_engine_get_number": (ctypes.c_int, [eng]),
+        "mpm_engine_get_particles": (ctypes.c_int, [eng, c_float_p]),
+        "mpm_engine_get_positions": (ctypes.c_int, [eng, c_float_p]),
+        "mpm_engine_get_sort": (ctypes.c_int, [eng, c_i32_p, c_i32_p]),
+        "mpm_engine_get_grid": (ctypes.c_int, [eng, c_float_p]),
+        "mpm_engine_get_stage_times": (ctypes.c_int, [eng, c_dbl_p, ctypes.c_int]),
+        "mpm_engine_get_counters": (ctypes.c_int, [eng, c_i64_p, ctypes.c_int]),
+        "mpm_engine_debug_p2g": (ctypes.c_int, [eng, c_float_p]),
+        "mpm_engine_debug_grid_update": (ctypes.c_int, [eng, c_float_p, c_float_p]),
+        "mpm_engine_debug_g2p": (ctypes.c_int, [eng, c_float_p]),
+        "mpm_scene_fill": (
+            ctypes.c_int,
+            [ctypes.c_int, c_float_p, c_float_p, ctypes.c_uint64, c_float_p, ctypes.c_int64, c_i64_p],
+        ),
+        "mpm_mesh_uv_sphere": (ctypes.c_int, [c_float_p, ctypes.c_float, ctypes.c_int, ctypes.c_int, c_float_p, c_int_p]),
+        "mpm_mesh_u_channel": (ctypes.c_int, [c_float_p, c_float_p, ctypes.c_float, ctypes.c_float, c_float_p, c_int_p]),
+        "mpm_mesh_box": (ctypes.c_int, [c_float_p, c_float_p, c_float_p, c_int_p]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != MPM_OK:
+        msg = load_library().mpm_last_error().decode(errors="replace")
+        raise MPMError(f"libmpm error {rc}: {msg}")
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _f32(a, shape=None) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+# ---- configuration ----
+
+def make_config(**kw) -> SimConfig:
+    """SimConfig from keyword arguments (reference key names); GRID_* derived."""
+    c = SimConfig()
+    defaults = dict(SIZE_X=64, SIZE_Y=64, SIZE_Z=64, PADDING=3, GRAVITY=9.8, DT=1e-3, GAMMA=-7.0,
+                    COMPRESSION=10.0, RESOLUTION=1.0, SUBSTEPS=20, SDF_RESOLUTION=64)
+    defaults.update(kw)
+    for k, v in defaults.items():
+        setattr(c, k, v)
+    _check(load_library().mpm_config_derive(ctypes.byref(c)))
+    return c
+
+
+def _parse(fn, arg: bytes, cap: int = 64):
+    cfg = SimConfig()
+    waters = (ConfigObject * cap)()
+    objects = (ConfigObject * cap)()
+    nw, no = ctypes.c_int(0), ctypes.c_int(0)
+    _check(fn(arg, ctypes.byref(cfg), waters, ctypes.byref(nw), objects, ctypes.byref(no), cap))
+
+    def conv(arr, n):
+        return [dict(model_path=o.model_path.decode(), scale=o.scale, x=o.x, y=o.y, z=o.z)
+                for o in list(arr)[: min(n, cap)]]
+
+    return cfg, conv(waters, nw.value), conv(objects, no.value)
+
+
+def parse_config_file(path) -> tuple[SimConfig, list, list]:
+    """GameConfigData::setConfigDataFromFile + initDeviceParams: (config, waters, objects)."""
+    return _parse(load_library().mpm_config_parse_file, str(path).encode())
+
+
+def parse_config_string(text: str) -> tuple[SimConfig, list, list]:
+    return _parse(load_library().mpm_config_parse_string, text.encode())
+
+
+def config_get_int(text: str, key: str) -> int:
+    v = ctypes.c_int64(0)
+    _check(load_library().mpm_config_get_int(text.encode(), key.encode(), ctypes.byref(v)))
+    return v.value
+
+
+# ---- scene generators ----
+
+def scene_fill(shape: str, lo, hi, seed: int) -> np.ndarray:
+    """Jittered 8-per-cell particles as a (26, n) float32 array."""
+    lib = load_library()
+    code = {"box": 0, "sphere": 1}[shape]
+    lo_a = _f32(lo, (3,))
+    hi_a = _f32(np.resize(np.asarray(hi, dtype=np.float32), 3), (3,))
+    cnt = ctypes.c_int64(0)
+    _check(lib.mpm_scene_fill(code, _fptr(lo_a), _fptr(hi_a), seed, None, 0, ctypes.byref(cnt)))
+    n = cnt.value
+    out = np.zeros((PARTICLE_ATTRS, n), dtype=np.float32)
+    cnt = ctypes.c_int64(0)
+    _check(lib.mpm_scene_fill(code, _fptr(lo_a), _fptr(hi_a), seed, _fptr(out), n, ctypes.byref(cnt)))
+    assert cnt.value == n
+    return out
+
+
+def _mesh(fn, *args) -> np.ndarray:
+    n = ctypes.c_int(0)
+    _check(fn(*args, None, ctypes.byref(n)))
+    out = np.zeros((n.value, 9), dtype=np.float32)
+    _check(fn(*args, _fptr(out), ctypes.byref(n)))
+    return out
+
+
+def mesh_uv_sphere(centre, radius: float, slices: int, stacks: int) -> np.ndarray:
+    c = _f32(centre, (3,))
+    return _mesh(load_library().mpm_mesh_uv_sphere, _fptr(c), ctypes.c_float(radius), slices, stacks)
+
+
+def mesh_u_channel(lo, hi, wall: float, seg: float) -> np.ndarray:
+    lo_a, hi_a = _f32(lo, (3,)), _f32(hi, (3,))
+    return _mesh(load_library().mpm_mesh_u_channel, _fptr(lo_a), _fptr(hi_a), ctypes.c_float(wall), ctypes.c_float(seg))
+
+
+def mesh_box(lo, hi) -> np.ndarray:
+    lo_a, hi_a = _f32(lo, (3,)), _f32(hi, (3,))
+    return _mesh(load_library().mpm_mesh_box, _fptr(lo_a), _fptr(hi_a))
+
+
+# ---- engine ----
+
+STAGES = ("p2g", "g2p", "g2p2g", "grid_update", "resort")
+
+
+class Engine:
+    """The reference's Engine (engine.h:14-43) over libmpm.so.
+
+    ``init(n, buffer)`` copies the caller's 26*n float buffer (the reference
+    takes ownership of it instead); everything else keeps the reference's
+    names and meaning.
+    """
+
+    def __init__(self, cfg: SimConfig, **options):
+        self._lib = load_library()
+        self.cfg = cfg
+        self._h = ctypes.c_void_p()
+        _check(self._lib.mpm_engine_create(ctypes.byref(cfg), ctypes.byref(self._h)))
+        for k, v in options.items():
+            self.set_option(k, v)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.mpm_engine_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_option(self, key: str, value: int) -> None:
+        _check(self._lib.mpm_engine_set_option(self._h, key.upper().encode(), int(value)))
+
+    def get_option(self, key: str) -> int:
+        v = ctypes.c_int64(0)
+        _check(self._lib.mpm_engine_get_option(self._h, key.upper().encode(), ctypes.byref(v)))
+        return v.value
+
+    # Engine::init (engine.cu:45-52)
+    def init(self, n: int, buffer) -> None:
+        buf = _f32(buffer).reshape(-1)
+        if buf.size != PARTICLE_ATTRS * n:
+            raise ValueError(f"buffer has {buf.size} floats, expected {PARTICLE_ATTRS * n}")
+        _check(self._lib.mpm_engine_init_particles(self._h, n, _fptr(buf)))
+
+    # Engine::initSDF (engine.cu:105-114)
+    def initSDF(self, triangles) -> None:
+        t = _f32(triangles).reshape(-1, 9)
+        _check(self._lib.mpm_engine_init_sdf_from_triangles(self._h, _fptr(t) if t.size else None, t.shape[0]))
+
+    def set_sdf(self, sdf) -> None:
+        s = _f32(sdf).reshape(-1)
+        if s.size != self.cfg.nodes:
+            raise ValueError("sdf must have SIZE_X*SIZE_Y*SIZE_Z values")
+        _check(self._lib.mpm_engine_set_sdf(self._h, _fptr(s)))
+
+    def get_sdf(self) -> np.ndarray:
+        out = np.empty(self.cfg.nodes, dtype=np.float32)
+        _check(self._lib.mpm_engine_get_sdf(self._h, _fptr(out)))
+        return out
+
+    # Engine::step (engine.cu:54-70)
+    def step(self) -> None:
+        _check(self._lib.mpm_engine_step(self._h))
+
+    def steps(self, k: int) -> None:
+        _check(self._lib.mpm_engine_steps(self._h, int(k)))
+
+    def synchronize(self) -> None:
+        _check(self._lib.mpm_engine_synchronize(self._h))
+
+    def getNumber(self) -> int:
+        return int(self._lib.mpm_engine_get_number(self._h))
+
+    def getParticles(self) -> np.ndarray:
+        n = self.getNumber()
+        out = np.empty((PARTICLE_ATTRS, n), dtype=np.float32)
+        _check(self._lib.mpm_engine_get_particles(self._h, _fptr(out)))
+        return out
+
+    def getPositions(self) -> np.ndarray:
+        n = self.getNumber()
+        out = np.empty((n, 3), dtype=np.float32)
+        _check(self._lib.mpm_engine_get_positions(self._h, _fptr(out)))
+        return out
+
+    def getGrid(self) -> np.ndarray:
+        out = np.empty((4, self.cfg.nodes), dtype=np.float32)
+        _check(self._lib.mpm_engine_get_grid(self._h, _fptr(out)))
+        return out
+
+    # Engine::sortParticles (engine.cu:92-102): (sorted keys, permutation)
+    def sortParticles(self) -> tuple[np.ndarray, np.ndarray]:
+        n = self.getNumber()
+        keys = np.empty(n, dtype=np.int32)
+        perm = np.empty(n, dtype=np.int32)
+        _check(self._lib.mpm_engine_get_sort(
+            self._h, keys.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            perm.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return keys, perm
+
+    def stage_times(self) -> dict:
+        out = np.zeros(2 * len(STAGES), dtype=np.float64)
+        _check(self._lib.mpm_engine_get_stage_times(
+            self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(STAGES)))
+        return {s: (float(out[2 * i]), int(out[2 * i + 1])) for i, s in enumerate(STAGES)}
+
+    def counters(self) -> dict:
+        out = np.zeros(3, dtype=np.int64)
+        _check(self._lib.mpm_engine_get_counters(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 3))
+        return {"chunks": int(out[0]), "tile_overflow_particles": int(out[1]), "active_blocks": int(out[2])}
+
+    def debug_p2g(self) -> np.ndarray:
+        out = np.empty((4, self.cfg.nodes), dtype=np.float32)
+        _check(self._lib.mpm_engine_debug_p2g(self._h, _fptr(out)))
+        return out
+
+    def debug_grid_update(self, grid4) -> np.ndarray:
+        g = _f32(grid4).reshape(4, -1)
+        out = np.empty_like(g)
+        _check(self._lib.mpm_engine_debug_grid_update(self._h, _fptr(g), _fptr(out)))
+        return out
+
+    def debug_g2p(self, grid4) -> None:
+        g = _f32(grid4).reshape(4, -1)
+        _check(self._lib.mpm_engine_debug_g2p(self._h, _fptr(g)))
+
+
+def gpu_available() -> bool:
+    """True when a HIP device is visible (torch is used only to ask)."""
+    try:
+        import torch
+
+        return bool(torch.cuda.is_available())
+    except Exception:
+        return False
+
+
+from . import scenes  # noqa: E402,F401

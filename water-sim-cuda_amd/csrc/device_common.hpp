@@ -1,0 +1,161 @@
+// device_common.hpp -- device-side math and layout shared by the HIP kernels.
+//
+// The per-particle and per-node arithmetic restates the reference's device
+// helpers (common.cuh:131-378) with the same operation order; the layout and
+// work decomposition are this engine's own (chunked Morton-sorted particles,
+// per-workgroup LDS tiles, 4^3 activity flags).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mpm {
+
+// Device-side copy of the scalar config.  Passed by value as a kernel argument
+// (the reference keeps it in __constant__ d_config, config.cu:3).
+struct DevCfg {
+    int sx, sy, sz;  // SIZE_X/Y/Z
+    int sxy;         // SIZE_X*SIZE_Y
+    uint32_t pad;    // PADDING
+    float gravity, dt, gamma, compression;
+    float vol, mass;  // per-particle V = RESOLUTION^3 and m = 1 (initFMatrices)
+    int fbx, fby, fbz;  // activity-flag blocks (4^3 nodes) per axis
+};
+
+// Particle planes: attribute-major, attr*stride + slot.  Attributes 0..23 of
+// the reference layout (pos, vel, C, F); VOL and MASS are engine scalars.
+enum : int {
+    A_PX = 0, A_PY, A_PZ, A_VX, A_VY, A_VZ,
+    A_C0, A_C1, A_C2, A_C3, A_C4, A_C5, A_C6, A_C7, A_C8,
+    A_F0, A_F1, A_F2, A_F3, A_F4, A_F5, A_F6, A_F7, A_F8,
+    A_STATE = 24
+};
+
+// Chunk = up to CHUNK particles of one 4^3-cell Morton block, processed by one
+// transfer workgroup.  Particle of in-chunk rank q lives in slot
+// chunk*CHUNK + (q % S)*64 + q / S, S = ceil(count/64): a wave's 64 lanes hold
+// particles S ranks apart, so they mostly sit in different cells and their
+// LDS atomics hit different nodes.
+struct ChunkDesc {
+    int start;       // first index of this chunk in the sorted order
+    int count;       // particles in the chunk
+    int ox, oy, oz;  // node origin of the chunk's LDS tile
+    int pad0, pad1, pad2;
+};
+
+constexpr int BLOCK_CELLS = 4;                             // chunk block edge, cells
+constexpr int BLOCK_SHIFT = 6;                             // log2(4^3): Morton key -> block
+constexpr int TILE_MARGIN = 1;                             // drift allowance, nodes
+constexpr int TILE_D = BLOCK_CELLS + 3 + 2 * TILE_MARGIN;  // 9 nodes per axis
+constexpr int TILE_N = TILE_D * TILE_D * TILE_D;           // 729
+constexpr int TRANSFER_THREADS = 256;
+
+__device__ __forceinline__ int gidx(const DevCfg& c, int x, int y, int z) { return z * c.sxy + y * c.sx + x; }
+
+// isInBounds, common.cuh:212-218 (negative coordinates compare as large unsigned).
+__device__ __forceinline__ bool inb(const DevCfg& c, int x, int y, int z) {
+    return (unsigned)x < (unsigned)c.sx && (unsigned)y < (unsigned)c.sy && (unsigned)z < (unsigned)c.sz;
+}
+
+__device__ __forceinline__ int flag_block(const DevCfg& c, int x, int y, int z) {
+    return ((z >> 2) * c.fby + (y >> 2)) * c.fbx + (x >> 2);
+}
+
+// spline, common.cuh:205-210.  `0.75 - x*x` rounds the float product first and
+// the double difference is exactly the float difference, so float arithmetic
+// without contraction reproduces it; the second branch squares an exact
+// difference and halves it, which float also reproduces.
+__device__ __forceinline__ float spline(float x) {
+#pragma clang fp contract(off)
+    float ax = fabsf(x);
+    if (ax < 0.5f) return 0.75f - x * x;
+    if (ax < 1.5f) {
+        float t = 1.5f - ax;
+        return 0.5f * (t * t);
+    }
+    return 0.0f;
+}
+
+// (int)(x - 0.5f): C truncation, kernels.cu:14, :204.
+__device__ __forceinline__ int base_of(float x) { return (int)(x - 0.5f); }
+
+// det3x3, common.cuh:250-253.
+__device__ __forceinline__ float det3(const float* A) {
+    return A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+           A[2] * (A[3] * A[7] - A[4] * A[6]);
+}
+
+// expandBits / calculateMorton, common.cuh:277-284, :304-307.
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+__device__ __forceinline__ uint32_t key_coord(float p) {
+    float v = fmaxf(0.0f, p);
+    return v < 4294967296.0f ? (uint32_t)v : 0xFFFFFFFFu;
+}
+// setKeys, kernels.cu:361-371.  Returned with the sign bit flipped so that an
+// unsigned sort orders the keys as the reference's signed-int sort does.
+__device__ __forceinline__ uint32_t sort_key(float x, float y, float z) {
+    uint32_t k = (expand_bits(key_coord(z)) << 2) | (expand_bits(key_coord(y)) << 1) | expand_bits(key_coord(x));
+    return k ^ 0x80000000u;
+}
+// Inverse of expand_bits on the 10 low interleaved bits.
+__device__ __forceinline__ uint32_t compact_bits(uint32_t v) {
+    v &= 0x49249249u;
+    v = (v ^ (v >> 2)) & 0xC30C30C3u;
+    v = (v ^ (v >> 4)) & 0x0F00F00Fu;
+    v = (v ^ (v >> 8)) & 0xFF0000FFu;
+    v = (v ^ (v >> 16)) & 0x000003FFu;
+    return v;
+}
+
+// getSDF, common.cuh:136-167: trilinear on node values, floor cells, clamped.
+__device__ __forceinline__ float sdf_at(const DevCfg& c, const float* __restrict__ sdf, float px, float py, float pz) {
+    int i = (int)floorf(px), j = (int)floorf(py), k = (int)floorf(pz);
+    float tx = px - (float)i, ty = py - (float)j, tz = pz - (float)k;
+    int x0 = max(0, min(i, c.sx - 1)), x1 = max(0, min(i + 1, c.sx - 1));
+    int y0 = max(0, min(j, c.sy - 1)), y1 = max(0, min(j + 1, c.sy - 1));
+    int z0 = max(0, min(k, c.sz - 1)), z1 = max(0, min(k + 1, c.sz - 1));
+    float n0 = sdf[gidx(c, x0, y0, z0)], n1 = sdf[gidx(c, x1, y0, z0)];
+    float n2 = sdf[gidx(c, x0, y1, z0)], n3 = sdf[gidx(c, x1, y1, z0)];
+    float n4 = sdf[gidx(c, x0, y0, z1)], n5 = sdf[gidx(c, x1, y0, z1)];
+    float n6 = sdf[gidx(c, x0, y1, z1)], n7 = sdf[gidx(c, x1, y1, z1)];
+    float a0 = n0 * (1.0f - tx) + n1 * tx, a1 = n2 * (1.0f - tx) + n3 * tx;
+    float a2 = n4 * (1.0f - tx) + n5 * tx, a3 = n6 * (1.0f - tx) + n7 * tx;
+    float b0 = a0 * (1.0f - ty) + a1 * ty, b1 = a2 * (1.0f - ty) + a3 * ty;
+    return b0 * (1.0f - tz) + b1 * tz;
+}
+
+// calculateNormal, common.cuh:169-182.
+__device__ __forceinline__ void sdf_normal(const DevCfg& c, const float* __restrict__ sdf, float px, float py, float pz,
+                                           float& nx, float& ny, float& nz) {
+    const float eps = 0.1f;
+    nx = sdf_at(c, sdf, px + eps, py, pz) - sdf_at(c, sdf, px - eps, py, pz);
+    ny = sdf_at(c, sdf, px, py + eps, pz) - sdf_at(c, sdf, px, py - eps, pz);
+    nz = sdf_at(c, sdf, px, py, pz + eps) - sdf_at(c, sdf, px, py, pz - eps);
+    float len = sqrtf(nx * nx + ny * ny + nz * nz) + 1e-6f;
+    nx /= len; ny /= len; nz /= len;
+}
+
+// getNormalAtNode, common.cuh:195-203.
+__device__ __forceinline__ void node_normal(const DevCfg& c, const float* __restrict__ sdf, int x, int y, int z,
+                                            float& nx, float& ny, float& nz) {
+    nx = sdf[gidx(c, min(x + 1, c.sx - 1), y, z)] - sdf[gidx(c, max(x - 1, 0), y, z)];
+    ny = sdf[gidx(c, x, min(y + 1, c.sy - 1), z)] - sdf[gidx(c, x, max(y - 1, 0), z)];
+    nz = sdf[gidx(c, x, y, min(z + 1, c.sz - 1))] - sdf[gidx(c, x, y, max(z - 1, 0))];
+    float len = sqrtf(nx * nx + ny * ny + nz * nz) + 1e-9f;
+    nx /= len; ny /= len; nz /= len;
+}
+
+__device__ __forceinline__ void atomic_add_global(float* p, float v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_add_lds(float* p, float v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+}  // namespace mpm

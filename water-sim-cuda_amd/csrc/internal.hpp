@@ -1,0 +1,98 @@
+// internal.hpp -- host-side declarations shared by the engine and the kernel
+// translation units (launchers only; no kernels are visible across files).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "device_common.hpp"
+
+namespace mpm {
+
+// Grid buffer: four planes of G floats (mass, x, y, z).  While it accumulates
+// P2G the planes hold mass and momentum; after the grid update the x/y/z
+// planes hold velocity (as the reference's Grid, common.cuh:110-121).
+struct GridBuf {
+    float* plane[4];
+    uint8_t* flags;  // one byte per 4^3 node block: touched by the last P2G into this buffer
+};
+
+enum TransferMode : int { XFER_P2G = 2, XFER_G2P = 1, XFER_G2P2G = 3 };
+
+struct TransferArgs {
+    DevCfg c;
+    float* part;          // particle planes, attr*stride + slot
+    size_t stride;
+    const ChunkDesc* chunks;
+    int nchunks;
+    int chunk_cap;        // slots per chunk
+    int per_xcd;          // ceil(nchunks / 8)
+    const float* vel[3];  // grid velocity planes read by G2P
+    float* acc[4];        // grid mass/momentum planes written by P2G
+    uint8_t* acc_flags;
+    const float* sdf;
+    unsigned int* overflow;  // particles that took the global-atomic path
+};
+
+void launch_transfer(int mode, const TransferArgs& a, hipStream_t s);
+
+// Grid update over the 4^3 blocks flagged in `m`, zeroing the blocks flagged in
+// `v_clear` (the previous velocity buffer) and clearing those flags.
+void launch_grid_update(const DevCfg& c, GridBuf m, GridBuf v_clear, bool clear_v, const float* sdf, int nblocks,
+                        hipStream_t s);
+
+void launch_make_sdf(const DevCfg& c, const float* tris, int nt, float* sdf, hipStream_t s);
+void launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
+void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t s);
+void launch_init_F(float* soa26, int n, hipStream_t s);
+
+// Export of the slot layout into the caller's order.
+void launch_export_soa26(const ChunkDesc* ch, int nchunks, int chunk_cap, const float* part, size_t stride,
+                         const int* orig, int n, float vol, float mass, float* out26, hipStream_t s);
+void launch_export_positions(const ChunkDesc* ch, int nchunks, int chunk_cap, const float* part, size_t stride,
+                             const int* orig, int n, float* out_planes3, hipStream_t s);
+
+// ---- sorting and chunking ----
+struct SortScratch {
+    uint32_t* keys[2];
+    uint32_t* vals[2];
+    uint32_t* hist;      // 256 * ntiles
+    uint32_t* partial;   // scan partials
+    uint32_t* counters;  // [0] = or, [1] = and, [2] = nseg, [3] = nchunks, ...
+    int* heads;          // n + 1
+    int* seg_start;      // n + 1
+    int* seg_nch;        // n + 1
+    size_t cap;          // elements
+    size_t hist_cap;     // elements of hist
+    size_t partial_cap;
+};
+
+size_t radix_hist_elems(size_t n);
+size_t scan_partial_elems(size_t n);
+
+// Keys of the particles in the current slot layout, written densely at
+// chunk.start + rank (vals = slot).
+void launch_keys_slots(const ChunkDesc* ch, int nchunks, int chunk_cap, const float* part, size_t stride,
+                       uint32_t* keys, uint32_t* vals, hipStream_t s);
+// Keys of caller-ordered position planes (vals = index).
+void launch_keys_linear(const float* px, const float* py, const float* pz, int n, uint32_t* keys, uint32_t* vals,
+                        hipStream_t s);
+// Stable LSD radix sort of (keys, vals) pairs; result in keys[r], vals[r] with
+// r returned (0 or 1).  Passes over key bytes on which all keys agree are
+// skipped (a stable sort by a constant digit is the identity).
+int radix_sort_pairs(SortScratch& sc, int n, hipStream_t s);
+// Exclusive scan of n uint32 values; total written to *total_dev.
+void scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* partial, uint32_t* total_dev,
+                        hipStream_t s);
+// Splits the sorted keys into chunks (one 4^3 Morton block per chunk, at most
+// chunk_cap particles).  Writes up to chunk_limit descriptors; the chunk count
+// (possibly above the limit) lands in sc.counters[3].
+void build_chunks(SortScratch& sc, int r, int n, int chunk_cap, ChunkDesc* chunks, int chunk_limit, hipStream_t s);
+// Gathers the particles into the chunked slot layout.  attr_mask selects the
+// planes moved (bit a = attribute a).
+void launch_reorder(const ChunkDesc* ch, int nchunks, int chunk_cap, const uint32_t* vals_sorted, const float* src,
+                    size_t src_stride, const int* src_orig, float* dst, size_t dst_stride, int* dst_orig,
+                    uint32_t attr_mask, hipStream_t s);
+
+}  // namespace mpm

@@ -1,0 +1,234 @@
+// kernels_grid.hip -- grid update, mesh SDF, state export and small fills.
+#include "internal.hpp"
+
+namespace mpm {
+namespace {
+
+// gridUpdate, kernels.cu:144-196, restricted to the 4^3 node blocks that the
+// last P2G touched.  One wave per block (lane = node, x fastest).  The same
+// launch zeroes the blocks of the previous velocity buffer that its own P2G
+// had touched, so that buffer is all-zero again when it next accumulates.
+__global__ __launch_bounds__(256) void k_grid_update(DevCfg c, GridBuf m, GridBuf v, int clear_v,
+                                                     const float* __restrict__ sdf, int nblocks) {
+    const int blk = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (blk >= nblocks) return;
+    const int lane = threadIdx.x & 63;
+    const bool fm = m.flags[blk] != 0;
+    const bool fv = clear_v && v.flags[blk] != 0;
+    if (!fm && !fv) return;
+    const int bx = blk % c.fbx, by = (blk / c.fbx) % c.fby, bz = blk / (c.fbx * c.fby);
+    const int x = bx * 4 + (lane & 3), y = by * 4 + ((lane >> 2) & 3), z = bz * 4 + (lane >> 4);
+    if (inb(c, x, y, z)) {
+        const int g = gidx(c, x, y, z);
+        if (fv) {
+            v.plane[0][g] = 0.0f;
+            v.plane[1][g] = 0.0f;
+            v.plane[2][g] = 0.0f;
+            v.plane[3][g] = 0.0f;
+        }
+        if (fm) {
+            const float mass = m.plane[0][g];
+            float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+            if (mass > 1e-9f) {
+                const float dist = sdf[g];
+                vx = m.plane[1][g] / mass;
+                vy = m.plane[2][g] / mass - c.gravity * c.dt;
+                vz = m.plane[3][g] / mass;
+                if (dist < 0.0f) {
+                    float nx, ny, nz;
+                    node_normal(c, sdf, x, y, z, nx, ny, nz);
+                    float vn = vx * nx + vy * ny + vz * nz;
+                    if (vn < 0.0f) {
+                        vx -= nx * vn; vy -= ny * vn; vz -= nz * vn;
+                    }
+                }
+                if ((uint32_t)x < c.pad && vx < 0) vx = 0.0f;
+                if ((uint32_t)y < c.pad && vy < 0) vy = 0.0f;
+                if ((uint32_t)z < c.pad && vz < 0) vz = 0.0f;
+                if ((uint32_t)x > (uint32_t)c.sx - 1u - c.pad && vx > 0) vx = 0.0f;
+                if ((uint32_t)y > (uint32_t)c.sy - 1u - c.pad && vy > 0) vy = 0.0f;
+                if ((uint32_t)z > (uint32_t)c.sz - 1u - c.pad && vz > 0) vz = 0.0f;
+            }
+            m.plane[1][g] = vx;
+            m.plane[2][g] = vy;
+            m.plane[3][g] = vz;
+        }
+    }
+    if (fv && lane == 0) v.flags[blk] = 0;
+}
+
+// makeSDF, kernels.cu:282-300, one thread per node.  The triangle index is
+// uniform across the wave, so the 36-byte triangle loads are scalar loads.
+// Winding by summed solid angles (calculateSolidAngle, common.cuh:310-323)
+// and the minimum point-triangle distance (common.cuh:325-378).
+__device__ __forceinline__ float point_tri_dist2(float px, float py, float pz, const float* t) {
+    const float ax = t[0], ay = t[1], az = t[2];
+    const float abx = t[3] - ax, aby = t[4] - ay, abz = t[5] - az;
+    const float acx = t[6] - ax, acy = t[7] - ay, acz = t[8] - az;
+    const float apx = px - ax, apy = py - ay, apz = pz - az;
+    const float d1 = abx * apx + aby * apy + abz * apz;
+    const float d2 = acx * apx + acy * apy + acz * apz;
+    if (d1 <= 0.0f && d2 <= 0.0f) return apx * apx + apy * apy + apz * apz;
+    const float bpx = px - t[3], bpy = py - t[4], bpz = pz - t[5];
+    const float d3 = abx * bpx + aby * bpy + abz * bpz;
+    const float d4 = acx * bpx + acy * bpy + acz * bpz;
+    if (d3 >= 0.0f && d4 <= d3) return bpx * bpx + bpy * bpy + bpz * bpz;
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        const float v = d1 / (d1 - d3);
+        const float qx = px - (ax + v * abx), qy = py - (ay + v * aby), qz = pz - (az + v * abz);
+        return qx * qx + qy * qy + qz * qz;
+    }
+    const float cpx = px - t[6], cpy = py - t[7], cpz = pz - t[8];
+    const float d5 = abx * cpx + aby * cpy + abz * cpz;
+    const float d6 = acx * cpx + acy * cpy + acz * cpz;
+    if (d6 >= 0.0f && d5 <= d6) return cpx * cpx + cpy * cpy + cpz * cpz;
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        const float w = d2 / (d2 - d6);
+        const float qx = px - (ax + w * acx), qy = py - (ay + w * acy), qz = pz - (az + w * acz);
+        return qx * qx + qy * qy + qz * qz;
+    }
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        const float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        const float qx = px - (t[3] + w * (t[6] - t[3]));
+        const float qy = py - (t[4] + w * (t[7] - t[4]));
+        const float qz = pz - (t[5] + w * (t[8] - t[5]));
+        return qx * qx + qy * qy + qz * qz;
+    }
+    const float denom = 1.0f / (va + vb + vc);
+    const float v = vb * denom, w = vc * denom;
+    const float qx = px - (ax + abx * v + acx * w);
+    const float qy = py - (ay + aby * v + acy * w);
+    const float qz = pz - (az + abz * v + acz * w);
+    return qx * qx + qy * qy + qz * qz;
+}
+
+__device__ __forceinline__ float solid_angle(float px, float py, float pz, const float* t) {
+    const float ax = t[0] - px, ay = t[1] - py, az = t[2] - pz;
+    const float bx = t[3] - px, by = t[4] - py, bz = t[5] - pz;
+    const float cx = t[6] - px, cy = t[7] - py, cz = t[8] - pz;
+    const float la = norm3df(ax, ay, az), lb = norm3df(bx, by, bz), lc = norm3df(cx, cy, cz);
+    const float det = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
+    const float div = (la * lb * lc) + (ax * bx + ay * by + az * bz) * lc + (ax * cx + ay * cy + az * cz) * lb +
+                      (bx * cx + by * cy + bz * cz) * la;
+    return 2.0f * atan2f(det, div);
+}
+
+__global__ __launch_bounds__(256) void k_make_sdf(DevCfg c, const float* __restrict__ tris, int nt,
+                                                  float* __restrict__ sdf) {
+    const int G = c.sxy * c.sz;
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= G) return;
+    const int x = idx % c.sx, y = (idx / c.sx) % c.sy, z = idx / c.sxy;
+    const float px = (float)x, py = (float)y, pz = (float)z;
+    float wn = 0.0f, mind = 1e30f;
+    for (int t = 0; t < nt; ++t) {
+        const float* tri = tris + 9 * (size_t)t;
+        wn += solid_angle(px, py, pz, tri);
+        const float d2 = point_tri_dist2(px, py, pz, tri);
+        if (d2 < mind) mind = d2;
+    }
+    wn = wn / (4.0f * 3.1415f);
+    sdf[idx] = (wn > 0.5f ? -1.0f : 1.0f) * sqrtf(mind);
+}
+
+__global__ void k_fill_f32(float* p, size_t n, float v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u8(uint8_t* p, size_t n, uint8_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// initFMatrices, kernels.cu:349-358 (diagonal of F; V and m are engine scalars).
+__global__ void k_init_F(float* soa26, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    soa26[(size_t)A_F0 * n + i] = 1.0f;
+    soa26[(size_t)A_F4 * n + i] = 1.0f;
+    soa26[(size_t)A_F8 * n + i] = 1.0f;
+}
+
+__global__ __launch_bounds__(256) void k_export_soa26(const ChunkDesc* __restrict__ ch, int chunk_cap,
+                                                      const float* __restrict__ part, size_t stride,
+                                                      const int* __restrict__ orig, int n, float vol, float mass,
+                                                      float* __restrict__ out) {
+    const ChunkDesc d = ch[blockIdx.x];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int S = (d.count + 63) >> 6;
+    for (int j = wave; j < S; j += 4) {
+        if (lane * S + j >= d.count) continue;
+        const size_t slot = (size_t)blockIdx.x * chunk_cap + (size_t)j * 64 + lane;
+        const int id = orig[slot];
+        for (int a = 0; a < A_STATE; ++a) out[(size_t)a * n + id] = part[(size_t)a * stride + slot];
+        out[(size_t)24 * n + id] = vol;
+        out[(size_t)25 * n + id] = mass;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_export_positions(const ChunkDesc* __restrict__ ch, int chunk_cap,
+                                                          const float* __restrict__ part, size_t stride,
+                                                          const int* __restrict__ orig, int n,
+                                                          float* __restrict__ out) {
+    const ChunkDesc d = ch[blockIdx.x];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int S = (d.count + 63) >> 6;
+    for (int j = wave; j < S; j += 4) {
+        if (lane * S + j >= d.count) continue;
+        const size_t slot = (size_t)blockIdx.x * chunk_cap + (size_t)j * 64 + lane;
+        const int id = orig[slot];
+        out[id] = part[A_PX * stride + slot];
+        out[(size_t)n + id] = part[A_PY * stride + slot];
+        out[(size_t)2 * n + id] = part[A_PZ * stride + slot];
+    }
+}
+
+}  // namespace
+
+void launch_grid_update(const DevCfg& c, GridBuf m, GridBuf v_clear, bool clear_v, const float* sdf, int nblocks,
+                        hipStream_t s) {
+    if (nblocks <= 0) return;
+    hipLaunchKernelGGL(k_grid_update, dim3((nblocks + 3) / 4), dim3(256), 0, s, c, m, v_clear, clear_v ? 1 : 0, sdf,
+                       nblocks);
+}
+
+void launch_make_sdf(const DevCfg& c, const float* tris, int nt, float* sdf, hipStream_t s) {
+    const int G = c.sxy * c.sz;
+    hipLaunchKernelGGL(k_make_sdf, dim3((G + 255) / 256), dim3(256), 0, s, c, tris, nt, sdf);
+}
+
+void launch_fill_f32(float* p, size_t n, float v, hipStream_t s) {
+    if (!n) return;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+}
+
+void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t s) {
+    if (!n) return;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+}
+
+void launch_init_F(float* soa26, int n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_init_F, dim3((n + 255) / 256), dim3(256), 0, s, soa26, n);
+}
+
+void launch_export_soa26(const ChunkDesc* ch, int nchunks, int chunk_cap, const float* part, size_t stride,
+                         const int* orig, int n, float vol, float mass, float* out26, hipStream_t s) {
+    if (nchunks <= 0) return;
+    hipLaunchKernelGGL(k_export_soa26, dim3(nchunks), dim3(256), 0, s, ch, chunk_cap, part, stride, orig, n, vol, mass,
+                       out26);
+}
+
+void launch_export_positions(const ChunkDesc* ch, int nchunks, int chunk_cap, const float* part, size_t stride,
+                             const int* orig, int n, float* out_planes3, hipStream_t s) {
+    if (nchunks <= 0) return;
+    hipLaunchKernelGGL(k_export_positions, dim3(nchunks), dim3(256), 0, s, ch, chunk_cap, part, stride, orig, n,
+                       out_planes3);
+}
+
+}  // namespace mpm

@@ -1,0 +1,298 @@
+// kernels_transfer.hip -- particle<->grid transfer kernels.
+//
+// One workgroup per chunk (<= CHUNK particles of one 4^3-cell Morton block).
+// The workgroup stages the grid velocity of its block's neighbourhood in an
+// LDS tile (G2P reads) and accumulates its P2G contributions in an LDS arena
+// of the same footprint, which it flushes with one global float atomic per
+// node and channel.  Particles whose stencil leaves the tile (they drifted
+// more than TILE_MARGIN nodes out of their block since the last re-sort, or
+// sit far outside the grid) fall back to global loads/atomics.
+//
+// Three modes share the code:
+//   XFER_P2G   : p2GTransferScatter, kernels.cu:6-142
+//   XFER_G2P   : g2PTransfer, kernels.cu:199-280
+//   XFER_G2P2G : G2P of substep s followed, in registers, by P2G of substep
+//                s+1 into the other grid buffer.  Between substeps only
+//                position and F then travel through HBM; v and C stay in
+//                registers (they are written out by the final XFER_G2P).
+#include "internal.hpp"
+
+namespace mpm {
+namespace {
+
+struct Particle {
+    float x, y, z;
+    float vx, vy, vz;
+    float C[9];
+    float F[9];
+};
+
+__device__ __forceinline__ void weights(float p, int b, float w[3], float d[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float r = p - (float)(b + i);  // pPos - (pos + i), kernels.cu:54, :211
+        w[i] = spline(r);
+        d[i] = -r;                     // dist = node - pPos
+    }
+}
+
+// ---- G2P (kernels.cu:199-280) ----
+template <class Fetch>
+__device__ __forceinline__ void g2p_gather(Fetch fetch, const float wx[3], const float wy[3], const float wz[3],
+                                           const float dx[3], const float dy[3], const float dz[3], float vel[3],
+                                           float C[9]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                float3 v = fetch(i, j, k);
+                float weight = wx[i] * wy[j] * wz[k];
+                vel[0] += weight * v.x;
+                vel[1] += weight * v.y;
+                vel[2] += weight * v.z;
+                float term = 4.0f * weight;
+                C[0] += term * v.x * dx[i]; C[1] += term * v.x * dy[j]; C[2] += term * v.x * dz[k];
+                C[3] += term * v.y * dx[i]; C[4] += term * v.y * dy[j]; C[5] += term * v.y * dz[k];
+                C[6] += term * v.z * dx[i]; C[7] += term * v.z * dy[j]; C[8] += term * v.z * dz[k];
+            }
+}
+
+// F update, J floor, advection and collider projection (kernels.cu:237-279).
+__device__ __forceinline__ void g2p_finish(const DevCfg& c, const float* __restrict__ sdf, Particle& p,
+                                           const float vel_in[3], const float C[9]) {
+    float A[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) A[i] = c.dt * C[i];
+    A[0] = 1.0f + A[0]; A[4] = 1.0f + A[4]; A[8] = 1.0f + A[8];
+    float nF[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            nF[3 * r + q] = A[3 * r] * p.F[q] + A[3 * r + 1] * p.F[3 + q] + A[3 * r + 2] * p.F[6 + q];
+    float J = det3(nF);
+    if (J < 0.3f) {
+        float scale = powf(0.3f / J, 1.0f / 3.0f);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) nF[i] *= scale;
+    }
+    float vx = vel_in[0], vy = vel_in[1], vz = vel_in[2];
+    float nx_ = p.x + vx * c.dt, ny_ = p.y + vy * c.dt, nz_ = p.z + vz * c.dt;
+    float dist = sdf_at(c, sdf, nx_, ny_, nz_);
+    if (dist < 0.0f) {
+        float nx, ny, nz;
+        sdf_normal(c, sdf, nx_, ny_, nz_, nx, ny, nz);
+        nx_ -= nx * dist; ny_ -= ny * dist; nz_ -= nz * dist;
+        float vn = vx * nx + vy * ny + vz * nz;
+        if (vn < 0.0f) {
+            vx -= nx * vn; vy -= ny * vn; vz -= nz * vn;
+        }
+    }
+    p.x = nx_; p.y = ny_; p.z = nz_;
+    p.vx = vx; p.vy = vy; p.vz = vz;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) p.C[i] = C[i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) p.F[i] = nF[i];
+}
+
+// ---- P2G (kernels.cu:14-119) ----
+template <class Scatter>
+__device__ __forceinline__ void p2g_scatter(const DevCfg& c, const Particle& p, int bx, int by, int bz,
+                                            Scatter scatter) {
+    float J = det3(p.F);
+    J = fmaxf(0.9f, fminf(J, 1.1f));
+    float pressure = c.compression * (powf(J, c.gamma) - 1.0f);
+    if (pressure < 0.0f) pressure = 0.0f;
+    float volume = c.vol * J;
+    float multi = 4.0f * c.dt * volume * pressure / c.mass;
+    float C[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) C[i] = p.C[i];
+    C[0] += multi; C[4] += multi; C[8] += multi;
+    float wx[3], wy[3], wz[3], dx[3], dy[3], dz[3];
+    weights(p.x, bx, wx, dx);
+    weights(p.y, by, wy, dy);
+    weights(p.z, bz, wz, dz);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                float weight = wx[i] * wy[j] * wz[k];
+                float wm = c.mass * weight;
+                float cx = C[0] * dx[i] + C[1] * dy[j] + C[2] * dz[k];
+                float cy = C[3] * dx[i] + C[4] * dy[j] + C[5] * dz[k];
+                float cz = C[6] * dx[i] + C[7] * dy[j] + C[8] * dz[k];
+                scatter(i, j, k, wm, wm * (p.vx + cx), wm * (p.vy + cy), wm * (p.vz + cz));
+            }
+}
+
+__device__ __forceinline__ void load_particle(const float* __restrict__ part, size_t stride, size_t slot, Particle& p,
+                                              bool full) {
+    p.x = part[A_PX * stride + slot];
+    p.y = part[A_PY * stride + slot];
+    p.z = part[A_PZ * stride + slot];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) p.F[i] = part[(A_F0 + i) * stride + slot];
+    if (full) {
+        p.vx = part[A_VX * stride + slot];
+        p.vy = part[A_VY * stride + slot];
+        p.vz = part[A_VZ * stride + slot];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) p.C[i] = part[(A_C0 + i) * stride + slot];
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
+    constexpr bool DO_G2P = (MODE & XFER_G2P) != 0;
+    constexpr bool DO_P2G = (MODE & XFER_P2G) != 0;
+    const int b = blockIdx.x;
+    const int ci = (b & 7) * a.per_xcd + (b >> 3);  // XCD-contiguous chunk ranges
+    if (ci >= a.nchunks) return;
+    const ChunkDesc d = a.chunks[ci];
+    const DevCfg& c = a.c;
+
+    __shared__ float4 vt[DO_G2P ? TILE_N : 1];
+    __shared__ float ar[DO_P2G ? 4 * TILE_N : 1];
+
+    const int tid = threadIdx.x;
+    if constexpr (DO_G2P) {
+        for (int i = tid; i < TILE_N; i += TRANSFER_THREADS) {
+            int lx = i % TILE_D, ly = (i / TILE_D) % TILE_D, lz = i / (TILE_D * TILE_D);
+            int x = d.ox + lx, y = d.oy + ly, z = d.oz + lz;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (inb(c, x, y, z)) {
+                int g = gidx(c, x, y, z);
+                v.x = a.vel[0][g];
+                v.y = a.vel[1][g];
+                v.z = a.vel[2][g];
+            }
+            vt[i] = v;
+        }
+    }
+    if constexpr (DO_P2G) {
+        for (int i = tid; i < 4 * TILE_N; i += TRANSFER_THREADS) ar[i] = 0.0f;
+    }
+    __syncthreads();
+
+    const int wave = tid >> 6, lane = tid & 63;
+    const int S = (d.count + 63) >> 6;
+    for (int j = wave; j < S; j += TRANSFER_THREADS / 64) {
+        const int q = lane * S + j;
+        if (q >= d.count) continue;
+        const size_t slot = (size_t)ci * a.chunk_cap + (size_t)j * 64 + lane;
+        Particle p;
+        load_particle(a.part, a.stride, slot, p, !DO_G2P);
+
+        if constexpr (DO_G2P) {
+            const int bx = base_of(p.x), by = base_of(p.y), bz = base_of(p.z);
+            float wx[3], wy[3], wz[3], dx[3], dy[3], dz[3];
+            weights(p.x, bx, wx, dx);
+            weights(p.y, by, wy, dy);
+            weights(p.z, bz, wz, dz);
+            float vel[3] = {0.f, 0.f, 0.f};
+            float C[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            const int lx = bx - d.ox, ly = by - d.oy, lz = bz - d.oz;
+            const bool in_tile = (unsigned)lx <= (unsigned)(TILE_D - 3) && (unsigned)ly <= (unsigned)(TILE_D - 3) &&
+                                 (unsigned)lz <= (unsigned)(TILE_D - 3);
+            if (in_tile) {
+                const float4* base = vt + (lx + TILE_D * ly + TILE_D * TILE_D * lz);
+                g2p_gather(
+                    [&](int i, int jj, int k) {
+                        float4 v = base[i + TILE_D * jj + TILE_D * TILE_D * k];
+                        return make_float3(v.x, v.y, v.z);
+                    },
+                    wx, wy, wz, dx, dy, dz, vel, C);
+            } else {
+                g2p_gather(
+                    [&](int i, int jj, int k) {
+                        int x = bx + i, y = by + jj, z = bz + k;
+                        if (!inb(c, x, y, z)) return make_float3(0.f, 0.f, 0.f);
+                        int g = gidx(c, x, y, z);
+                        return make_float3(a.vel[0][g], a.vel[1][g], a.vel[2][g]);
+                    },
+                    wx, wy, wz, dx, dy, dz, vel, C);
+            }
+            g2p_finish(c, a.sdf, p, vel, C);
+            a.part[A_PX * a.stride + slot] = p.x;
+            a.part[A_PY * a.stride + slot] = p.y;
+            a.part[A_PZ * a.stride + slot] = p.z;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) a.part[(A_F0 + i) * a.stride + slot] = p.F[i];
+            if constexpr (!DO_P2G) {
+                a.part[A_VX * a.stride + slot] = p.vx;
+                a.part[A_VY * a.stride + slot] = p.vy;
+                a.part[A_VZ * a.stride + slot] = p.vz;
+#pragma unroll
+                for (int i = 0; i < 9; ++i) a.part[(A_C0 + i) * a.stride + slot] = p.C[i];
+            }
+        }
+
+        if constexpr (DO_P2G) {
+            const int bx = base_of(p.x), by = base_of(p.y), bz = base_of(p.z);
+            const int lx = bx - d.ox, ly = by - d.oy, lz = bz - d.oz;
+            const bool in_tile = (unsigned)lx <= (unsigned)(TILE_D - 3) && (unsigned)ly <= (unsigned)(TILE_D - 3) &&
+                                 (unsigned)lz <= (unsigned)(TILE_D - 3);
+            if (in_tile) {
+                float* base = ar + (lx + TILE_D * ly + TILE_D * TILE_D * lz);
+                p2g_scatter(c, p, bx, by, bz, [&](int i, int jj, int k, float m, float mx, float my, float mz) {
+                    float* n = base + (i + TILE_D * jj + TILE_D * TILE_D * k);
+                    atomic_add_lds(n, m);
+                    atomic_add_lds(n + TILE_N, mx);
+                    atomic_add_lds(n + 2 * TILE_N, my);
+                    atomic_add_lds(n + 3 * TILE_N, mz);
+                });
+            } else {
+                atomicAdd(a.overflow, 1u);
+                p2g_scatter(c, p, bx, by, bz, [&](int i, int jj, int k, float m, float mx, float my, float mz) {
+                    int x = bx + i, y = by + jj, z = bz + k;
+                    if (!inb(c, x, y, z)) return;
+                    int g = gidx(c, x, y, z);
+                    atomic_add_global(a.acc[0] + g, m);
+                    atomic_add_global(a.acc[1] + g, mx);
+                    atomic_add_global(a.acc[2] + g, my);
+                    atomic_add_global(a.acc[3] + g, mz);
+                    a.acc_flags[flag_block(c, x, y, z)] = 1;
+                });
+            }
+        }
+    }
+
+    if constexpr (DO_P2G) {
+        __syncthreads();
+        for (int i = tid; i < TILE_N; i += TRANSFER_THREADS) {
+            float m = ar[i];
+            if (m > 1e-9f) {  // the reference's tile flush threshold, kernels.cu:132
+                int lx = i % TILE_D, ly = (i / TILE_D) % TILE_D, lz = i / (TILE_D * TILE_D);
+                int x = d.ox + lx, y = d.oy + ly, z = d.oz + lz;
+                if (inb(c, x, y, z)) {
+                    int g = gidx(c, x, y, z);
+                    atomic_add_global(a.acc[0] + g, m);
+                    atomic_add_global(a.acc[1] + g, ar[i + TILE_N]);
+                    atomic_add_global(a.acc[2] + g, ar[i + 2 * TILE_N]);
+                    atomic_add_global(a.acc[3] + g, ar[i + 3 * TILE_N]);
+                    a.acc_flags[flag_block(c, x, y, z)] = 1;
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+
+void launch_transfer(int mode, const TransferArgs& a, hipStream_t s) {
+    if (a.nchunks <= 0) return;
+    dim3 grid(8 * a.per_xcd), block(TRANSFER_THREADS);
+    switch (mode) {
+        case XFER_P2G: hipLaunchKernelGGL(k_transfer<XFER_P2G>, grid, block, 0, s, a); break;
+        case XFER_G2P: hipLaunchKernelGGL(k_transfer<XFER_G2P>, grid, block, 0, s, a); break;
+        default: hipLaunchKernelGGL(k_transfer<XFER_G2P2G>, grid, block, 0, s, a); break;
+    }
+}
+
+}  // namespace mpm

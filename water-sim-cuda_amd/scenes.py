@@ -1,0 +1,93 @@
+"""Benchmark and test scenes (seeded, generated; no reference counterpart).
+
+Each scene returns a dict with ``cfg`` (SimConfig), ``particles`` (26 x n
+float32, caller order), the collider as either ``triangles`` (T x 9, built
+into the SDF by makeSDF) or ``sdf`` (G floats, analytic), ``resort_every``,
+``frames`` and a short ``description``.
+
+Parameter conventions.  The reference subtracts GRAVITY*DT from v_y
+(kernels.cu:155) and computes pressure as COMPRESSION*(J^GAMMA - 1) clamped
+at zero with J = det F (kernels.cu:26-29).  Physical water under these
+formulas needs GRAVITY > 0 (downwards) and GAMMA < 0 (pressure under
+compression): ``physical=True`` uses GRAVITY = 9.8 and GAMMA = -7 (Tait's
+exponent 7 on density rho/rho0 = 1/J).  ``physical=False`` keeps the literal
+values GRAVITY = -9.8, GAMMA = 7 of the 64^3 sphere workload; parity does not
+depend on the choice because both sides receive the same values.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def _pkg():
+    import importlib
+
+    return importlib.import_module("water-sim-cuda_amd")
+
+
+def _params(physical: bool) -> dict:
+    if physical:
+        return dict(GRAVITY=9.8, GAMMA=-7.0, COMPRESSION=10.0)
+    return dict(GRAVITY=-9.8, GAMMA=7.0, COMPRESSION=10.0)
+
+
+def floor_sdf(cfg, top: float) -> np.ndarray:
+    """Analytic axis-aligned box floor: node SDF = y - top (negative below)."""
+    sz, sy, sx = cfg.grid_shape()
+    y = np.arange(sy, dtype=np.float32)
+    return np.broadcast_to((y - np.float32(top))[None, :, None], (sz, sy, sx)).astype(np.float32).reshape(-1)
+
+
+def sphere64(physical: bool = False, seed: int = 1, radius: float = 12.0) -> dict:
+    """64^3, PADDING 3, water sphere r=12 at (32,40,32), 8 ppc, DT 1e-3,
+    20 substeps x 100 frames, analytic floor at y = 4."""
+    m = _pkg()
+    cfg = m.make_config(SIZE_X=64, SIZE_Y=64, SIZE_Z=64, PADDING=3, DT=1e-3, RESOLUTION=1.0, SUBSTEPS=20,
+                        SDF_RESOLUTION=64, **_params(physical))
+    parts = m.scene_fill("sphere", (32.0, 40.0, 32.0), (radius,), seed)
+    return dict(name="sphere64", cfg=cfg, particles=parts, sdf=floor_sdf(cfg, 4.0), triangles=None,
+                resort_every=1, frames=100,
+                description="64^3 grid, water sphere r=12 at (32,40,32), 8 ppc, box floor SDF")
+
+
+def dambreak128(physical: bool = True, seed: int = 2) -> dict:
+    """128^3 dam break: 48x96x48-cell block against the x=4 wall (N = 1,769,472),
+    UV-sphere collider (32 slices x 33 stacks = 2,048 triangles, r = 20),
+    DT 5e-4, 20 substeps x 200 frames, re-sort every 10 substeps."""
+    m = _pkg()
+    cfg = m.make_config(SIZE_X=128, SIZE_Y=128, SIZE_Z=128, PADDING=3, DT=5e-4, RESOLUTION=1.0, SUBSTEPS=20,
+                        SDF_RESOLUTION=128, **_params(physical))
+    parts = m.scene_fill("box", (4.0, 4.0, 40.0), (52.0, 100.0, 88.0), seed)
+    tris = m.mesh_uv_sphere((88.3, 24.3, 64.3), 20.0, 32, 33)
+    return dict(name="dambreak128", cfg=cfg, particles=parts, sdf=None, triangles=tris, resort_every=10,
+                frames=200, description="128^3 dam break 48x96x48 cells, UV-sphere r=20 (2048 tris)")
+
+
+def channel256(physical: bool = True, seed: int = 3) -> dict:
+    """256^3: water boxes 96x128x96 and 64^3 cells (N = 11,534,336) above a
+    U-channel (wall 4 cells, ~10k triangles), DT 2.5e-4, 40 substeps x 100
+    frames.  This scene measures the HBM roofline."""
+    m = _pkg()
+    cfg = m.make_config(SIZE_X=256, SIZE_Y=256, SIZE_Z=256, PADDING=3, DT=2.5e-4, RESOLUTION=1.0, SUBSTEPS=40,
+                        SDF_RESOLUTION=256, **_params(physical))
+    a = m.scene_fill("box", (16.0, 104.0, 16.0), (112.0, 232.0, 112.0), seed)
+    b = m.scene_fill("box", (150.0, 120.0, 150.0), (214.0, 184.0, 214.0), seed + 1000)
+    parts = np.concatenate([a, b], axis=1)
+    tris = m.mesh_u_channel((6.3, 20.3, 6.3), (249.7, 90.3, 249.7), 4.0, 6.25)
+    return dict(name="channel256", cfg=cfg, particles=parts, sdf=None, triangles=tris, resort_every=10,
+                frames=100, description="256^3, boxes 96x128x96 + 64^3 cells into a U-channel mesh")
+
+
+def small_box(n_cells=(6, 6, 6), grid=32, physical: bool = True, seed: int = 7, dt: float = 1e-3) -> dict:
+    """A small jittered block for quick tests (8 * prod(n_cells) particles)."""
+    m = _pkg()
+    cfg = m.make_config(SIZE_X=grid, SIZE_Y=grid, SIZE_Z=grid, PADDING=3, DT=dt, RESOLUTION=1.0, SUBSTEPS=20,
+                        SDF_RESOLUTION=grid, **_params(physical))
+    lo = np.array([grid / 2 - n_cells[0] / 2, 8.0, grid / 2 - n_cells[2] / 2], dtype=np.float32)
+    hi = lo + np.array(n_cells, dtype=np.float32)
+    parts = m.scene_fill("box", lo, hi, seed)
+    return dict(name="small_box", cfg=cfg, particles=parts, sdf=floor_sdf(cfg, 4.0), triangles=None,
+                resort_every=1, frames=10, description=f"{grid}^3 block {n_cells}")
+
+
+SCENES = {"sphere64": sphere64, "dambreak128": dambreak128, "channel256": channel256}
