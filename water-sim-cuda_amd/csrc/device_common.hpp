@@ -34,8 +34,8 @@ enum : int {
 // Chunk = up to CHUNK particles of one 4^3-cell Morton block, processed by one
 // transfer workgroup.  Particle of in-chunk rank q lives in slot
 // chunk*CHUNK + (q % S)*64 + q / S, S = ceil(count/64): a wave's 64 lanes hold
-// particles S ranks apart, so they mostly sit in different cells and their
-// LDS atomics hit different nodes.
+// particles S ranks apart, so they mostly sit in different cells and update
+// different arena nodes at the same stencil offset.
 struct ChunkDesc {
     int start;       // first index of this chunk in the sorted order
     int count;       // particles in the chunk
@@ -48,7 +48,7 @@ constexpr int BLOCK_SHIFT = 6;                             // log2(4^3): Morton 
 constexpr int TILE_MARGIN = 1;                             // drift allowance, nodes
 constexpr int TILE_D = BLOCK_CELLS + 3 + 2 * TILE_MARGIN;  // 9 nodes per axis
 constexpr int TILE_N = TILE_D * TILE_D * TILE_D;           // 729
-constexpr int TRANSFER_THREADS = 256;
+constexpr int TRANSFER_THREADS = 64;  // one wave per chunk (wave-private LDS arena)
 
 __device__ __forceinline__ int gidx(const DevCfg& c, int x, int y, int z) { return z * c.sxy + y * c.sx + x; }
 
@@ -153,9 +153,6 @@ __device__ __forceinline__ void node_normal(const DevCfg& c, const float* __rest
 
 __device__ __forceinline__ void atomic_add_global(float* p, float v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void atomic_add_lds(float* p, float v) {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 }  // namespace mpm

@@ -1,12 +1,23 @@
 // kernels_transfer.hip -- particle<->grid transfer kernels.
 //
-// One workgroup per chunk (<= CHUNK particles of one 4^3-cell Morton block).
-// The workgroup stages the grid velocity of its block's neighbourhood in an
-// LDS tile (G2P reads) and accumulates its P2G contributions in an LDS arena
-// of the same footprint, which it flushes with one global float atomic per
-// node and channel.  Particles whose stencil leaves the tile (they drifted
-// more than TILE_MARGIN nodes out of their block since the last re-sort, or
-// sit far outside the grid) fall back to global loads/atomics.
+// One single-wave workgroup per chunk (<= CHUNK particles of one 4^3-cell
+// Morton block).  The wave stages the grid velocity of its block's
+// neighbourhood in an LDS tile (G2P reads) and accumulates its P2G
+// contributions in an LDS arena of the same footprint, which it flushes with
+// one global float atomic per node and channel.  Particles whose stencil
+// leaves the tile (they drifted more than TILE_MARGIN nodes out of their
+// block since the last re-sort, or sit far outside the grid) fall back to
+// global loads/atomics.
+//
+// Arena accumulation.  FP32 LDS atomics (ds_add_f32) run at about a third of
+// a lane per clock per CU on gfx950, some 40x below integer LDS atomics, so
+// the arena is updated with plain float4 read-modify-writes instead.  The
+// arena belongs to one wave; the 27 stencil nodes are visited in program
+// order and LDS executes one wave's operations in order, so a later offset
+// always sees an earlier offset's write.  Within one offset, lanes must hit
+// distinct nodes: lanes whose particles share a base node are found with a
+// ballot match and take turns.  The sums are the same float additions the
+// reference performs, in a different order.
 //
 // Three modes share the code:
 //   XFER_P2G   : p2GTransferScatter, kernels.cu:6-142
@@ -34,6 +45,20 @@ __device__ __forceinline__ void weights(float p, int b, float w[3], float d[3]) 
         w[i] = spline(r);
         d[i] = -r;                     // dist = node - pPos
     }
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
+
+// Lanes in `active` whose 10-bit value equals this lane's.
+__device__ __forceinline__ uint64_t match10(uint32_t v, uint64_t active) {
+    uint64_t m = active;
+#pragma unroll
+    for (int b = 0; b < 10; ++b) {
+        const bool bit = (v >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
+    }
+    return m;
 }
 
 // ---- G2P (kernels.cu:199-280) ----
@@ -99,34 +124,42 @@ __device__ __forceinline__ void g2p_finish(const DevCfg& c, const float* __restr
 }
 
 // ---- P2G (kernels.cu:14-119) ----
-template <class Scatter>
-__device__ __forceinline__ void p2g_scatter(const DevCfg& c, const Particle& p, int bx, int by, int bz,
-                                            Scatter scatter) {
+// Per-particle quantities: pressure-augmented affine matrix (kernels.cu:26-35),
+// weights and node offsets.
+struct P2GPrep {
+    float C[9];
+    float wx[3], wy[3], wz[3], dx[3], dy[3], dz[3];
+};
+
+__device__ __forceinline__ void p2g_prepare(const DevCfg& c, const Particle& p, int bx, int by, int bz, P2GPrep& q) {
     float J = det3(p.F);
     J = fmaxf(0.9f, fminf(J, 1.1f));
     float pressure = c.compression * (powf(J, c.gamma) - 1.0f);
     if (pressure < 0.0f) pressure = 0.0f;
     float volume = c.vol * J;
     float multi = 4.0f * c.dt * volume * pressure / c.mass;
-    float C[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) C[i] = p.C[i];
-    C[0] += multi; C[4] += multi; C[8] += multi;
-    float wx[3], wy[3], wz[3], dx[3], dy[3], dz[3];
-    weights(p.x, bx, wx, dx);
-    weights(p.y, by, wy, dy);
-    weights(p.z, bz, wz, dz);
+    for (int i = 0; i < 9; ++i) q.C[i] = p.C[i];
+    q.C[0] += multi; q.C[4] += multi; q.C[8] += multi;
+    weights(p.x, bx, q.wx, q.dx);
+    weights(p.y, by, q.wy, q.dy);
+    weights(p.z, bz, q.wz, q.dz);
+}
+
+// Node contributions in the reference's loop order (x outer, z inner).
+template <class Scatter>
+__device__ __forceinline__ void p2g_scatter(const DevCfg& c, const Particle& p, const P2GPrep& q, Scatter scatter) {
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                float weight = wx[i] * wy[j] * wz[k];
+                float weight = q.wx[i] * q.wy[j] * q.wz[k];
                 float wm = c.mass * weight;
-                float cx = C[0] * dx[i] + C[1] * dy[j] + C[2] * dz[k];
-                float cy = C[3] * dx[i] + C[4] * dy[j] + C[5] * dz[k];
-                float cz = C[6] * dx[i] + C[7] * dy[j] + C[8] * dz[k];
+                float cx = q.C[0] * q.dx[i] + q.C[1] * q.dy[j] + q.C[2] * q.dz[k];
+                float cy = q.C[3] * q.dx[i] + q.C[4] * q.dy[j] + q.C[5] * q.dz[k];
+                float cz = q.C[6] * q.dx[i] + q.C[7] * q.dy[j] + q.C[8] * q.dz[k];
                 scatter(i, j, k, wm, wm * (p.vx + cx), wm * (p.vy + cy), wm * (p.vz + cz));
             }
 }
@@ -147,6 +180,11 @@ __device__ __forceinline__ void load_particle(const float* __restrict__ part, si
     }
 }
 
+__device__ __forceinline__ bool stencil_in_tile(int lx, int ly, int lz) {
+    return (unsigned)lx <= (unsigned)(TILE_D - 3) && (unsigned)ly <= (unsigned)(TILE_D - 3) &&
+           (unsigned)lz <= (unsigned)(TILE_D - 3);
+}
+
 template <int MODE>
 __global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
     constexpr bool DO_G2P = (MODE & XFER_G2P) != 0;
@@ -158,11 +196,11 @@ __global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
     const DevCfg& c = a.c;
 
     __shared__ float4 vt[DO_G2P ? TILE_N : 1];
-    __shared__ float ar[DO_P2G ? 4 * TILE_N : 1];
+    __shared__ float4 ar[DO_P2G ? TILE_N : 1];  // mass, momentum x, y, z
 
-    const int tid = threadIdx.x;
+    const int lane = threadIdx.x;
     if constexpr (DO_G2P) {
-        for (int i = tid; i < TILE_N; i += TRANSFER_THREADS) {
+        for (int i = lane; i < TILE_N; i += TRANSFER_THREADS) {
             int lx = i % TILE_D, ly = (i / TILE_D) % TILE_D, lz = i / (TILE_D * TILE_D);
             int x = d.ox + lx, y = d.oy + ly, z = d.oz + lz;
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -176,106 +214,121 @@ __global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
         }
     }
     if constexpr (DO_P2G) {
-        for (int i = tid; i < 4 * TILE_N; i += TRANSFER_THREADS) ar[i] = 0.0f;
+        for (int i = lane; i < TILE_N; i += TRANSFER_THREADS) ar[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     __syncthreads();
 
-    const int wave = tid >> 6, lane = tid & 63;
     const int S = (d.count + 63) >> 6;
-    for (int j = wave; j < S; j += TRANSFER_THREADS / 64) {
+    for (int j = 0; j < S; ++j) {
         const int q = lane * S + j;
-        if (q >= d.count) continue;
+        const bool valid = q < d.count;
         const size_t slot = (size_t)ci * a.chunk_cap + (size_t)j * 64 + lane;
         Particle p;
-        load_particle(a.part, a.stride, slot, p, !DO_G2P);
-
-        if constexpr (DO_G2P) {
-            const int bx = base_of(p.x), by = base_of(p.y), bz = base_of(p.z);
-            float wx[3], wy[3], wz[3], dx[3], dy[3], dz[3];
-            weights(p.x, bx, wx, dx);
-            weights(p.y, by, wy, dy);
-            weights(p.z, bz, wz, dz);
-            float vel[3] = {0.f, 0.f, 0.f};
-            float C[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            const int lx = bx - d.ox, ly = by - d.oy, lz = bz - d.oz;
-            const bool in_tile = (unsigned)lx <= (unsigned)(TILE_D - 3) && (unsigned)ly <= (unsigned)(TILE_D - 3) &&
-                                 (unsigned)lz <= (unsigned)(TILE_D - 3);
-            if (in_tile) {
-                const float4* base = vt + (lx + TILE_D * ly + TILE_D * TILE_D * lz);
-                g2p_gather(
-                    [&](int i, int jj, int k) {
-                        float4 v = base[i + TILE_D * jj + TILE_D * TILE_D * k];
-                        return make_float3(v.x, v.y, v.z);
-                    },
-                    wx, wy, wz, dx, dy, dz, vel, C);
-            } else {
-                g2p_gather(
-                    [&](int i, int jj, int k) {
-                        int x = bx + i, y = by + jj, z = bz + k;
-                        if (!inb(c, x, y, z)) return make_float3(0.f, 0.f, 0.f);
-                        int g = gidx(c, x, y, z);
-                        return make_float3(a.vel[0][g], a.vel[1][g], a.vel[2][g]);
-                    },
-                    wx, wy, wz, dx, dy, dz, vel, C);
-            }
-            g2p_finish(c, a.sdf, p, vel, C);
-            a.part[A_PX * a.stride + slot] = p.x;
-            a.part[A_PY * a.stride + slot] = p.y;
-            a.part[A_PZ * a.stride + slot] = p.z;
+        if (valid) {
+            load_particle(a.part, a.stride, slot, p, !DO_G2P);
+            if constexpr (DO_G2P) {
+                const int bx = base_of(p.x), by = base_of(p.y), bz = base_of(p.z);
+                float wx[3], wy[3], wz[3], dx[3], dy[3], dz[3];
+                weights(p.x, bx, wx, dx);
+                weights(p.y, by, wy, dy);
+                weights(p.z, bz, wz, dz);
+                float vel[3] = {0.f, 0.f, 0.f};
+                float C[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                const int lx = bx - d.ox, ly = by - d.oy, lz = bz - d.oz;
+                if (stencil_in_tile(lx, ly, lz)) {
+                    const float4* base = vt + (lx + TILE_D * ly + TILE_D * TILE_D * lz);
+                    g2p_gather(
+                        [&](int i, int jj, int k) {
+                            float4 v = base[i + TILE_D * jj + TILE_D * TILE_D * k];
+                            return make_float3(v.x, v.y, v.z);
+                        },
+                        wx, wy, wz, dx, dy, dz, vel, C);
+                } else {
+                    g2p_gather(
+                        [&](int i, int jj, int k) {
+                            int x = bx + i, y = by + jj, z = bz + k;
+                            if (!inb(c, x, y, z)) return make_float3(0.f, 0.f, 0.f);
+                            int g = gidx(c, x, y, z);
+                            return make_float3(a.vel[0][g], a.vel[1][g], a.vel[2][g]);
+                        },
+                        wx, wy, wz, dx, dy, dz, vel, C);
+                }
+                g2p_finish(c, a.sdf, p, vel, C);
+                a.part[A_PX * a.stride + slot] = p.x;
+                a.part[A_PY * a.stride + slot] = p.y;
+                a.part[A_PZ * a.stride + slot] = p.z;
 #pragma unroll
-            for (int i = 0; i < 9; ++i) a.part[(A_F0 + i) * a.stride + slot] = p.F[i];
-            if constexpr (!DO_P2G) {
-                a.part[A_VX * a.stride + slot] = p.vx;
-                a.part[A_VY * a.stride + slot] = p.vy;
-                a.part[A_VZ * a.stride + slot] = p.vz;
+                for (int i = 0; i < 9; ++i) a.part[(A_F0 + i) * a.stride + slot] = p.F[i];
+                if constexpr (!DO_P2G) {
+                    a.part[A_VX * a.stride + slot] = p.vx;
+                    a.part[A_VY * a.stride + slot] = p.vy;
+                    a.part[A_VZ * a.stride + slot] = p.vz;
 #pragma unroll
-                for (int i = 0; i < 9; ++i) a.part[(A_C0 + i) * a.stride + slot] = p.C[i];
+                    for (int i = 0; i < 9; ++i) a.part[(A_C0 + i) * a.stride + slot] = p.C[i];
+                }
             }
         }
 
         if constexpr (DO_P2G) {
-            const int bx = base_of(p.x), by = base_of(p.y), bz = base_of(p.z);
-            const int lx = bx - d.ox, ly = by - d.oy, lz = bz - d.oz;
-            const bool in_tile = (unsigned)lx <= (unsigned)(TILE_D - 3) && (unsigned)ly <= (unsigned)(TILE_D - 3) &&
-                                 (unsigned)lz <= (unsigned)(TILE_D - 3);
-            if (in_tile) {
-                float* base = ar + (lx + TILE_D * ly + TILE_D * TILE_D * lz);
-                p2g_scatter(c, p, bx, by, bz, [&](int i, int jj, int k, float m, float mx, float my, float mz) {
-                    float* n = base + (i + TILE_D * jj + TILE_D * TILE_D * k);
-                    atomic_add_lds(n, m);
-                    atomic_add_lds(n + TILE_N, mx);
-                    atomic_add_lds(n + 2 * TILE_N, my);
-                    atomic_add_lds(n + 3 * TILE_N, mz);
-                });
-            } else {
-                atomicAdd(a.overflow, 1u);
-                p2g_scatter(c, p, bx, by, bz, [&](int i, int jj, int k, float m, float mx, float my, float mz) {
-                    int x = bx + i, y = by + jj, z = bz + k;
-                    if (!inb(c, x, y, z)) return;
-                    int g = gidx(c, x, y, z);
-                    atomic_add_global(a.acc[0] + g, m);
-                    atomic_add_global(a.acc[1] + g, mx);
-                    atomic_add_global(a.acc[2] + g, my);
-                    atomic_add_global(a.acc[3] + g, mz);
-                    a.acc_flags[flag_block(c, x, y, z)] = 1;
-                });
+            int bx = 0, by = 0, bz = 0;
+            bool in_tile = false;
+            uint32_t li = 0;
+            P2GPrep pq;
+            if (valid) {
+                bx = base_of(p.x); by = base_of(p.y); bz = base_of(p.z);
+                const int lx = bx - d.ox, ly = by - d.oy, lz = bz - d.oz;
+                in_tile = stencil_in_tile(lx, ly, lz);
+                li = in_tile ? (uint32_t)(lx + TILE_D * ly + TILE_D * TILE_D * lz) : 0u;
+                p2g_prepare(c, p, bx, by, bz, pq);
+                if (!in_tile) {
+                    atomicAdd(a.overflow, 1u);
+                    p2g_scatter(c, p, pq, [&](int i, int jj, int k, float m, float mx, float my, float mz) {
+                        int x = bx + i, y = by + jj, z = bz + k;
+                        if (!inb(c, x, y, z)) return;
+                        int g = gidx(c, x, y, z);
+                        atomic_add_global(a.acc[0] + g, m);
+                        atomic_add_global(a.acc[1] + g, mx);
+                        atomic_add_global(a.acc[2] + g, my);
+                        atomic_add_global(a.acc[3] + g, mz);
+                        a.acc_flags[flag_block(c, x, y, z)] = 1;
+                    });
+                }
+            }
+            // Lanes with distinct base nodes update the arena together; lanes
+            // sharing a base node take turns.
+            uint64_t todo = __ballot(in_tile);
+            while (todo) {
+                const uint64_t peers = match10(li, todo);
+                const bool mine = ((todo >> lane) & 1ull) && (peers & lanemask_lt(lane)) == 0ull;
+                const uint64_t done = __ballot(mine);
+                if (mine) {
+                    float4* base = ar + li;
+                    p2g_scatter(c, p, pq, [&](int i, int jj, int k, float m, float mx, float my, float mz) {
+                        float4* n = base + (i + TILE_D * jj + TILE_D * TILE_D * k);
+                        float4 v = *n;
+                        v.x += m; v.y += mx; v.z += my; v.w += mz;
+                        *n = v;
+                        asm volatile("" ::: "memory");  // keep the 27 updates in program order
+                    });
+                }
+                todo &= ~done;
             }
         }
     }
 
     if constexpr (DO_P2G) {
         __syncthreads();
-        for (int i = tid; i < TILE_N; i += TRANSFER_THREADS) {
-            float m = ar[i];
-            if (m > 1e-9f) {  // the reference's tile flush threshold, kernels.cu:132
+        for (int i = lane; i < TILE_N; i += TRANSFER_THREADS) {
+            const float4 v = ar[i];
+            if (v.x > 1e-9f) {  // the reference's tile flush threshold, kernels.cu:132
                 int lx = i % TILE_D, ly = (i / TILE_D) % TILE_D, lz = i / (TILE_D * TILE_D);
                 int x = d.ox + lx, y = d.oy + ly, z = d.oz + lz;
                 if (inb(c, x, y, z)) {
                     int g = gidx(c, x, y, z);
-                    atomic_add_global(a.acc[0] + g, m);
-                    atomic_add_global(a.acc[1] + g, ar[i + TILE_N]);
-                    atomic_add_global(a.acc[2] + g, ar[i + 2 * TILE_N]);
-                    atomic_add_global(a.acc[3] + g, ar[i + 3 * TILE_N]);
+                    atomic_add_global(a.acc[0] + g, v.x);
+                    atomic_add_global(a.acc[1] + g, v.y);
+                    atomic_add_global(a.acc[2] + g, v.z);
+                    atomic_add_global(a.acc[3] + g, v.w);
                     a.acc_flags[flag_block(c, x, y, z)] = 1;
                 }
             }
