@@ -45,10 +45,28 @@ struct ChunkDesc {
 
 constexpr int BLOCK_CELLS = 4;                             // chunk block edge, cells
 constexpr int BLOCK_SHIFT = 6;                             // log2(4^3): Morton key -> block
-constexpr int TILE_MARGIN = 1;                             // drift allowance, nodes
-constexpr int TILE_D = BLOCK_CELLS + 3 + 2 * TILE_MARGIN;  // 9 nodes per axis
-constexpr int TILE_N = TILE_D * TILE_D * TILE_D;           // 729
+// Drift allowance of the tiles, in nodes.  A particle's base node leaves its
+// block's range only after it moves half a cell out of the block, so 0 keeps
+// all particles in their tiles between re-sorts unless they move that far.
+constexpr int TILE_MARGIN = 0;
+constexpr int TILE_D = BLOCK_CELLS + 3 + 2 * TILE_MARGIN;  // 7 nodes per axis
+constexpr int TILE_N = TILE_D * TILE_D * TILE_D;           // 343
 constexpr int TRANSFER_THREADS = 64;  // one wave per chunk (wave-private LDS arena)
+
+// LDS slot pitches of the tile/arena (16-byte slots).  Lanes of one LDS access
+// hold particles of neighbouring cells (a 2x2x2 group per 8 lanes); pitches
+// of 2 and 4 modulo 8 in y and z put such a group on 8 distinct 16-byte bank
+// groups.
+#ifndef MPM_TILE_PY
+#define MPM_TILE_PY 10
+#endif
+#ifndef MPM_TILE_PZ
+#define MPM_TILE_PZ 76
+#endif
+constexpr int TILE_PY = MPM_TILE_PY;
+constexpr int TILE_PZ = MPM_TILE_PZ;
+constexpr int TILE_SLOTS = (TILE_D - 1) * (1 + TILE_PY + TILE_PZ) + 1;
+static_assert(TILE_PY >= TILE_D && TILE_PZ >= TILE_PY * TILE_D, "tile pitches must not overlap");
 
 __device__ __forceinline__ int gidx(const DevCfg& c, int x, int y, int z) { return z * c.sxy + y * c.sx + x; }
 
@@ -113,6 +131,15 @@ __device__ __forceinline__ uint32_t compact_bits(uint32_t v) {
     return v;
 }
 
+// Trilinear blend of the 8 corner values, in getSDF's order (common.cuh:154-166).
+__device__ __forceinline__ float trilerp(float n0, float n1, float n2, float n3, float n4, float n5, float n6,
+                                         float n7, float tx, float ty, float tz) {
+    float a0 = n0 * (1.0f - tx) + n1 * tx, a1 = n2 * (1.0f - tx) + n3 * tx;
+    float a2 = n4 * (1.0f - tx) + n5 * tx, a3 = n6 * (1.0f - tx) + n7 * tx;
+    float b0 = a0 * (1.0f - ty) + a1 * ty, b1 = a2 * (1.0f - ty) + a3 * ty;
+    return b0 * (1.0f - tz) + b1 * tz;
+}
+
 // getSDF, common.cuh:136-167: trilinear on node values, floor cells, clamped.
 __device__ __forceinline__ float sdf_at(const DevCfg& c, const float* __restrict__ sdf, float px, float py, float pz) {
     int i = (int)floorf(px), j = (int)floorf(py), k = (int)floorf(pz);
@@ -120,14 +147,9 @@ __device__ __forceinline__ float sdf_at(const DevCfg& c, const float* __restrict
     int x0 = max(0, min(i, c.sx - 1)), x1 = max(0, min(i + 1, c.sx - 1));
     int y0 = max(0, min(j, c.sy - 1)), y1 = max(0, min(j + 1, c.sy - 1));
     int z0 = max(0, min(k, c.sz - 1)), z1 = max(0, min(k + 1, c.sz - 1));
-    float n0 = sdf[gidx(c, x0, y0, z0)], n1 = sdf[gidx(c, x1, y0, z0)];
-    float n2 = sdf[gidx(c, x0, y1, z0)], n3 = sdf[gidx(c, x1, y1, z0)];
-    float n4 = sdf[gidx(c, x0, y0, z1)], n5 = sdf[gidx(c, x1, y0, z1)];
-    float n6 = sdf[gidx(c, x0, y1, z1)], n7 = sdf[gidx(c, x1, y1, z1)];
-    float a0 = n0 * (1.0f - tx) + n1 * tx, a1 = n2 * (1.0f - tx) + n3 * tx;
-    float a2 = n4 * (1.0f - tx) + n5 * tx, a3 = n6 * (1.0f - tx) + n7 * tx;
-    float b0 = a0 * (1.0f - ty) + a1 * ty, b1 = a2 * (1.0f - ty) + a3 * ty;
-    return b0 * (1.0f - tz) + b1 * tz;
+    return trilerp(sdf[gidx(c, x0, y0, z0)], sdf[gidx(c, x1, y0, z0)], sdf[gidx(c, x0, y1, z0)],
+                   sdf[gidx(c, x1, y1, z0)], sdf[gidx(c, x0, y0, z1)], sdf[gidx(c, x1, y0, z1)],
+                   sdf[gidx(c, x0, y1, z1)], sdf[gidx(c, x1, y1, z1)], tx, ty, tz);
 }
 
 // calculateNormal, common.cuh:169-182.

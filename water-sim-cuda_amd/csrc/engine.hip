@@ -216,6 +216,8 @@ int sort_and_reorder(mpm_engine* h, const float* src, size_t src_stride, const i
         if ((rc = dev_alloc(&h->chunks, (size_t)h->chunk_limit))) return rc;
     }
     const int dst = 1 - h->pcur;
+    if ((uint64_t)h->chunk_limit * (uint64_t)h->chunk_cap >= (1ull << 32))
+        return fail(MPM_ELIMIT, "particle slots exceed 2^32 (too many sparsely filled 4^3-cell blocks)");
     int rc = ensure_planes(h, dst, (size_t)h->chunk_limit * h->chunk_cap);
     if (rc) return rc;
     launch_reorder(h->chunks, h->nchunks, h->chunk_cap, h->sc.vals[r], src, src_stride, src_orig, h->planes[dst],

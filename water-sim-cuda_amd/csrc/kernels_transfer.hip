@@ -39,54 +39,98 @@ struct Particle {
 };
 
 __device__ __forceinline__ void weights(float p, int b, float w[3], float d[3]) {
+    // fr = p - b is exact, and for fr in [0.5, 1.5) each of the three spline
+    // branches is fixed, so the closed forms below equal spline(p - (b + i))
+    // bit for bit (every difference involved is exact).  Otherwise (particles
+    // below 0.5 or far outside the grid) the generic spline is evaluated.
+    const float fr = p - (float)b;
+    d[0] = -fr;
+    d[1] = -(fr - 1.0f);
+    d[2] = -(fr - 2.0f);
+    if (fr >= 0.5f && fr < 1.5f) {
+#pragma clang fp contract(off)
+        const float t0 = 1.5f - fr, t1 = fr - 1.0f, t2 = fr - 0.5f;
+        w[0] = 0.5f * (t0 * t0);
+        w[1] = 0.75f - t1 * t1;
+        w[2] = 0.5f * (t2 * t2);
+    } else {
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        float r = p - (float)(b + i);  // pPos - (pos + i), kernels.cu:54, :211
-        w[i] = spline(r);
-        d[i] = -r;                     // dist = node - pPos
+        for (int i = 0; i < 3; ++i) {
+            const float r = p - (float)(b + i);  // pPos - (pos + i), kernels.cu:54, :211
+            w[i] = spline(r);
+            d[i] = -r;                           // dist = node - pPos
+        }
     }
-}
-
-__device__ __forceinline__ uint64_t lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
-
-// Lanes in `active` whose 10-bit value equals this lane's.
-__device__ __forceinline__ uint64_t match10(uint32_t v, uint64_t active) {
-    uint64_t m = active;
-#pragma unroll
-    for (int b = 0; b < 10; ++b) {
-        const bool bit = (v >> b) & 1u;
-        const uint64_t bb = __ballot(bit);
-        m &= bit ? bb : ~bb;
-    }
-    return m;
 }
 
 // ---- G2P (kernels.cu:199-280) ----
+// vel = sum w v and C = 4 sum w v (x_i - x_p)^T over the 27 nodes.  The
+// sums are grouped by axis (z innermost, then y, then x), which needs about
+// 280 multiply-adds instead of the 650 of a node-by-node accumulation; the
+// additions happen in a different order than the reference's.
 template <class Fetch>
 __device__ __forceinline__ void g2p_gather(Fetch fetch, const float wx[3], const float wy[3], const float wz[3],
                                            const float dx[3], const float dy[3], const float dz[3], float vel[3],
                                            float C[9]) {
+    float wxd[3], wyd[3], wzd[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < 3; ++i) {
+        wxd[i] = wx[i] * dx[i];
+        wyd[i] = wy[i] * dy[i];
+        wzd[i] = wz[i] * dz[i];
+    }
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) {
+        float S[3] = {0.f, 0.f, 0.f}, Y[3] = {0.f, 0.f, 0.f}, T[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            float s[3] = {0.f, 0.f, 0.f}, t[3] = {0.f, 0.f, 0.f};
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                float3 v = fetch(i, j, k);
-                float weight = wx[i] * wy[j] * wz[k];
-                vel[0] += weight * v.x;
-                vel[1] += weight * v.y;
-                vel[2] += weight * v.z;
-                float term = 4.0f * weight;
-                C[0] += term * v.x * dx[i]; C[1] += term * v.x * dy[j]; C[2] += term * v.x * dz[k];
-                C[3] += term * v.y * dx[i]; C[4] += term * v.y * dy[j]; C[5] += term * v.y * dz[k];
-                C[6] += term * v.z * dx[i]; C[7] += term * v.z * dy[j]; C[8] += term * v.z * dz[k];
+                const float3 v = fetch(i, j, k);
+                s[0] += wz[k] * v.x; s[1] += wz[k] * v.y; s[2] += wz[k] * v.z;
+                t[0] += wzd[k] * v.x; t[1] += wzd[k] * v.y; t[2] += wzd[k] * v.z;
             }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                S[a] += wy[j] * s[a];
+                Y[a] += wyd[j] * s[a];
+                T[a] += wy[j] * t[a];
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            vel[a] += wx[i] * S[a];
+            C[3 * a + 0] += wxd[i] * S[a];
+            C[3 * a + 1] += wx[i] * Y[a];
+            C[3 * a + 2] += wx[i] * T[a];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) C[i] *= 4.0f;
+}
+
+// getSDF at p from the .w lanes of the LDS tile when the sample's 2x2x2 cell
+// lies inside the tile and needs no clamping (then both read the same node
+// values); otherwise from the global SDF.
+__device__ __forceinline__ float sdf_sample(const DevCfg& c, const float* __restrict__ sdf, const float4* vt, int ox,
+                                            int oy, int oz, float px, float py, float pz) {
+    const int i = (int)floorf(px), j = (int)floorf(py), k = (int)floorf(pz);
+    const int lx = i - ox, ly = j - oy, lz = k - oz;
+    if ((unsigned)lx <= (unsigned)(TILE_D - 2) && (unsigned)ly <= (unsigned)(TILE_D - 2) &&
+        (unsigned)lz <= (unsigned)(TILE_D - 2) && i >= 0 && j >= 0 && k >= 0 && i + 1 < c.sx && j + 1 < c.sy &&
+        k + 1 < c.sz) {
+        const float4* b = vt + (lx + TILE_PY * ly + TILE_PZ * lz);
+        return trilerp(b[0].w, b[1].w, b[TILE_PY].w, b[TILE_PY + 1].w, b[TILE_PZ].w, b[TILE_PZ + 1].w,
+                       b[TILE_PZ + TILE_PY].w, b[TILE_PZ + TILE_PY + 1].w, px - (float)i, py - (float)j,
+                       pz - (float)k);
+    }
+    return sdf_at(c, sdf, px, py, pz);
 }
 
 // F update, J floor, advection and collider projection (kernels.cu:237-279).
-__device__ __forceinline__ void g2p_finish(const DevCfg& c, const float* __restrict__ sdf, Particle& p,
-                                           const float vel_in[3], const float C[9]) {
+__device__ __forceinline__ void g2p_finish(const DevCfg& c, const float* __restrict__ sdf, const float4* vt, int ox,
+                                           int oy, int oz, Particle& p, const float vel_in[3], const float C[9]) {
     float A[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) A[i] = c.dt * C[i];
@@ -105,7 +149,7 @@ __device__ __forceinline__ void g2p_finish(const DevCfg& c, const float* __restr
     }
     float vx = vel_in[0], vy = vel_in[1], vz = vel_in[2];
     float nx_ = p.x + vx * c.dt, ny_ = p.y + vy * c.dt, nz_ = p.z + vz * c.dt;
-    float dist = sdf_at(c, sdf, nx_, ny_, nz_);
+    float dist = sdf_sample(c, sdf, vt, ox, oy, oz, nx_, ny_, nz_);
     if (dist < 0.0f) {
         float nx, ny, nz;
         sdf_normal(c, sdf, nx_, ny_, nz_, nx, ny, nz);
@@ -146,37 +190,53 @@ __device__ __forceinline__ void p2g_prepare(const DevCfg& c, const Particle& p, 
     weights(p.z, bz, q.wz, q.dz);
 }
 
-// Node contributions in the reference's loop order (x outer, z inner).
-template <class Scatter>
+// Node contributions m w (v + C' (x_i - x_p)) in the reference's loop order
+// (x outer, z inner), with the x and y parts of C' (x_i - x_p) shared by the
+// three nodes of a z column.  `scatter(i, j, k, contrib)` receives a functor
+// returning (m w, momentum) so it can evaluate it where it wants; with PIN each
+// z column's inputs pass through an empty asm inside the column's first
+// update, which keeps the compiler from hoisting all 27 contributions (108
+// registers) above the update chain.
+template <bool PIN, class Scatter>
 __device__ __forceinline__ void p2g_scatter(const DevCfg& c, const Particle& p, const P2GPrep& q, Scatter scatter) {
+    float mwz[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) mwz[k] = c.mass * q.wz[k];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
+        for (int j = 0; j < 3; ++j) {
+            float w = q.wx[i] * q.wy[j];
+            float e0 = p.vx + q.C[0] * q.dx[i] + q.C[1] * q.dy[j];
+            float e1 = p.vy + q.C[3] * q.dx[i] + q.C[4] * q.dy[j];
+            float e2 = p.vz + q.C[6] * q.dx[i] + q.C[7] * q.dy[j];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                float weight = q.wx[i] * q.wy[j] * q.wz[k];
-                float wm = c.mass * weight;
-                float cx = q.C[0] * q.dx[i] + q.C[1] * q.dy[j] + q.C[2] * q.dz[k];
-                float cy = q.C[3] * q.dx[i] + q.C[4] * q.dy[j] + q.C[5] * q.dz[k];
-                float cz = q.C[6] * q.dx[i] + q.C[7] * q.dy[j] + q.C[8] * q.dz[k];
-                scatter(i, j, k, wm, wm * (p.vx + cx), wm * (p.vy + cy), wm * (p.vz + cz));
+                scatter(i, j, k, [&]() {
+                    if constexpr (PIN) {
+                        if (k == 0) asm volatile("" : "+v"(w), "+v"(e0), "+v"(e1), "+v"(e2));
+                    }
+                    const float wm = w * mwz[k];
+                    return make_float4(wm, wm * (e0 + q.C[2] * q.dz[k]), wm * (e1 + q.C[5] * q.dz[k]),
+                                       wm * (e2 + q.C[8] * q.dz[k]));
+                });
             }
+        }
 }
 
-__device__ __forceinline__ void load_particle(const float* __restrict__ part, size_t stride, size_t slot, Particle& p,
+__device__ __forceinline__ void load_particle(const float* __restrict__ part, size_t stride, uint32_t slot, Particle& p,
                                               bool full) {
-    p.x = part[A_PX * stride + slot];
-    p.y = part[A_PY * stride + slot];
-    p.z = part[A_PZ * stride + slot];
+    p.x = (part + (size_t)A_PX * stride)[slot];
+    p.y = (part + (size_t)A_PY * stride)[slot];
+    p.z = (part + (size_t)A_PZ * stride)[slot];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) p.F[i] = part[(A_F0 + i) * stride + slot];
+    for (int i = 0; i < 9; ++i) p.F[i] = (part + (size_t)(A_F0 + i) * stride)[slot];
     if (full) {
-        p.vx = part[A_VX * stride + slot];
-        p.vy = part[A_VY * stride + slot];
-        p.vz = part[A_VZ * stride + slot];
+        p.vx = (part + (size_t)A_VX * stride)[slot];
+        p.vy = (part + (size_t)A_VY * stride)[slot];
+        p.vz = (part + (size_t)A_VZ * stride)[slot];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) p.C[i] = part[(A_C0 + i) * stride + slot];
+        for (int i = 0; i < 9; ++i) p.C[i] = (part + (size_t)(A_C0 + i) * stride)[slot];
     }
 }
 
@@ -195,8 +255,9 @@ __global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
     const ChunkDesc d = a.chunks[ci];
     const DevCfg& c = a.c;
 
-    __shared__ float4 vt[DO_G2P ? TILE_N : 1];
-    __shared__ float4 ar[DO_P2G ? TILE_N : 1];  // mass, momentum x, y, z
+    __shared__ float4 vt[DO_G2P ? TILE_SLOTS : 1];
+    __shared__ float4 ar[DO_P2G ? TILE_SLOTS : 1];  // mass, momentum x, y, z
+    __shared__ uint8_t claim[DO_P2G ? TILE_SLOTS : 1];
 
     const int lane = threadIdx.x;
     if constexpr (DO_G2P) {
@@ -209,23 +270,30 @@ __global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
                 v.x = a.vel[0][g];
                 v.y = a.vel[1][g];
                 v.z = a.vel[2][g];
+                v.w = a.sdf[g];
             }
-            vt[i] = v;
+            vt[lx + TILE_PY * ly + TILE_PZ * lz] = v;
         }
     }
     if constexpr (DO_P2G) {
-        for (int i = lane; i < TILE_N; i += TRANSFER_THREADS) ar[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int i = lane; i < TILE_SLOTS; i += TRANSFER_THREADS) ar[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     __syncthreads();
 
     const int S = (d.count + 63) >> 6;
+    // Round j handles rank lane*S + j.  The next round's particle is loaded
+    // while the current one is processed, hiding HBM latency.
+    const uint32_t slot0 = (uint32_t)ci * (uint32_t)a.chunk_cap + (uint32_t)lane;
+    Particle nxt;
+    bool nxt_valid = lane * S < d.count;
+    if (nxt_valid) load_particle(a.part, a.stride, slot0, nxt, !DO_G2P);
     for (int j = 0; j < S; ++j) {
-        const int q = lane * S + j;
-        const bool valid = q < d.count;
-        const size_t slot = (size_t)ci * a.chunk_cap + (size_t)j * 64 + lane;
-        Particle p;
+        const bool valid = nxt_valid;
+        const uint32_t slot = slot0 + (uint32_t)j * 64u;
+        Particle p = nxt;
+        nxt_valid = j + 1 < S && lane * S + j + 1 < d.count;
+        if (nxt_valid) load_particle(a.part, a.stride, slot + 64u, nxt, !DO_G2P);
         if (valid) {
-            load_particle(a.part, a.stride, slot, p, !DO_G2P);
             if constexpr (DO_G2P) {
                 const int bx = base_of(p.x), by = base_of(p.y), bz = base_of(p.z);
                 float wx[3], wy[3], wz[3], dx[3], dy[3], dz[3];
@@ -236,10 +304,10 @@ __global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
                 float C[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 const int lx = bx - d.ox, ly = by - d.oy, lz = bz - d.oz;
                 if (stencil_in_tile(lx, ly, lz)) {
-                    const float4* base = vt + (lx + TILE_D * ly + TILE_D * TILE_D * lz);
+                    const float4* base = vt + (lx + TILE_PY * ly + TILE_PZ * lz);
                     g2p_gather(
                         [&](int i, int jj, int k) {
-                            float4 v = base[i + TILE_D * jj + TILE_D * TILE_D * k];
+                            float4 v = base[i + TILE_PY * jj + TILE_PZ * k];
                             return make_float3(v.x, v.y, v.z);
                         },
                         wx, wy, wz, dx, dy, dz, vel, C);
@@ -253,18 +321,18 @@ __global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
                         },
                         wx, wy, wz, dx, dy, dz, vel, C);
                 }
-                g2p_finish(c, a.sdf, p, vel, C);
-                a.part[A_PX * a.stride + slot] = p.x;
-                a.part[A_PY * a.stride + slot] = p.y;
-                a.part[A_PZ * a.stride + slot] = p.z;
+                g2p_finish(c, a.sdf, vt, d.ox, d.oy, d.oz, p, vel, C);
+                (a.part + (size_t)A_PX * a.stride)[slot] = p.x;
+                (a.part + (size_t)A_PY * a.stride)[slot] = p.y;
+                (a.part + (size_t)A_PZ * a.stride)[slot] = p.z;
 #pragma unroll
-                for (int i = 0; i < 9; ++i) a.part[(A_F0 + i) * a.stride + slot] = p.F[i];
+                for (int i = 0; i < 9; ++i) (a.part + (size_t)(A_F0 + i) * a.stride)[slot] = p.F[i];
                 if constexpr (!DO_P2G) {
-                    a.part[A_VX * a.stride + slot] = p.vx;
-                    a.part[A_VY * a.stride + slot] = p.vy;
-                    a.part[A_VZ * a.stride + slot] = p.vz;
+                    (a.part + (size_t)A_VX * a.stride)[slot] = p.vx;
+                    (a.part + (size_t)A_VY * a.stride)[slot] = p.vy;
+                    (a.part + (size_t)A_VZ * a.stride)[slot] = p.vz;
 #pragma unroll
-                    for (int i = 0; i < 9; ++i) a.part[(A_C0 + i) * a.stride + slot] = p.C[i];
+                    for (int i = 0; i < 9; ++i) (a.part + (size_t)(A_C0 + i) * a.stride)[slot] = p.C[i];
                 }
             }
         }
@@ -278,40 +346,46 @@ __global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
                 bx = base_of(p.x); by = base_of(p.y); bz = base_of(p.z);
                 const int lx = bx - d.ox, ly = by - d.oy, lz = bz - d.oz;
                 in_tile = stencil_in_tile(lx, ly, lz);
-                li = in_tile ? (uint32_t)(lx + TILE_D * ly + TILE_D * TILE_D * lz) : 0u;
+                li = in_tile ? (uint32_t)(lx + TILE_PY * ly + TILE_PZ * lz) : 0u;
                 p2g_prepare(c, p, bx, by, bz, pq);
                 if (!in_tile) {
                     atomicAdd(a.overflow, 1u);
-                    p2g_scatter(c, p, pq, [&](int i, int jj, int k, float m, float mx, float my, float mz) {
+                    p2g_scatter<false>(c, p, pq, [&](int i, int jj, int k, auto contrib) {
                         int x = bx + i, y = by + jj, z = bz + k;
                         if (!inb(c, x, y, z)) return;
                         int g = gidx(c, x, y, z);
-                        atomic_add_global(a.acc[0] + g, m);
-                        atomic_add_global(a.acc[1] + g, mx);
-                        atomic_add_global(a.acc[2] + g, my);
-                        atomic_add_global(a.acc[3] + g, mz);
+                        const float4 d = contrib();
+                        atomic_add_global(a.acc[0] + g, d.x);
+                        atomic_add_global(a.acc[1] + g, d.y);
+                        atomic_add_global(a.acc[2] + g, d.z);
+                        atomic_add_global(a.acc[3] + g, d.w);
                         a.acc_flags[flag_block(c, x, y, z)] = 1;
                     });
                 }
             }
             // Lanes with distinct base nodes update the arena together; lanes
-            // sharing a base node take turns.
-            uint64_t todo = __ballot(in_tile);
-            while (todo) {
-                const uint64_t peers = match10(li, todo);
-                const bool mine = ((todo >> lane) & 1ull) && (peers & lanemask_lt(lane)) == 0ull;
-                const uint64_t done = __ballot(mine);
+            // sharing a base node take turns.  Each pending lane writes its
+            // id into the claim byte of its base slot and reads it back: one
+            // writer per slot wins (LDS executes the wave's write before its
+            // read), and the winners of a turn have distinct base slots.
+            bool pending = in_tile;
+            while (__ballot(pending)) {
+                if (pending) claim[li] = (uint8_t)lane;
+                asm volatile("" ::: "memory");
+                const bool mine = pending && claim[li] == (uint8_t)lane;
+                asm volatile("" ::: "memory");
                 if (mine) {
                     float4* base = ar + li;
-                    p2g_scatter(c, p, pq, [&](int i, int jj, int k, float m, float mx, float my, float mz) {
-                        float4* n = base + (i + TILE_D * jj + TILE_D * TILE_D * k);
+                    p2g_scatter<true>(c, p, pq, [&](int i, int jj, int k, auto contrib) {
+                        float4* n = base + (i + TILE_PY * jj + TILE_PZ * k);
                         float4 v = *n;
-                        v.x += m; v.y += mx; v.z += my; v.w += mz;
+                        const float4 d = contrib();
+                        v.x += d.x; v.y += d.y; v.z += d.z; v.w += d.w;
                         *n = v;
                         asm volatile("" ::: "memory");  // keep the 27 updates in program order
                     });
                 }
-                todo &= ~done;
+                pending = pending && !mine;
             }
         }
     }
@@ -319,9 +393,9 @@ __global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
     if constexpr (DO_P2G) {
         __syncthreads();
         for (int i = lane; i < TILE_N; i += TRANSFER_THREADS) {
-            const float4 v = ar[i];
+            const int lx = i % TILE_D, ly = (i / TILE_D) % TILE_D, lz = i / (TILE_D * TILE_D);
+            const float4 v = ar[lx + TILE_PY * ly + TILE_PZ * lz];
             if (v.x > 1e-9f) {  // the reference's tile flush threshold, kernels.cu:132
-                int lx = i % TILE_D, ly = (i / TILE_D) % TILE_D, lz = i / (TILE_D * TILE_D);
                 int x = d.ox + lx, y = d.oy + ly, z = d.oz + lz;
                 if (inb(c, x, y, z)) {
                     int g = gidx(c, x, y, z);
