@@ -35,6 +35,22 @@ BYTES_PER_PARTICLE = 96
 BYTES_PER_ACTIVE_NODE = 32
 
 
+def read_traffic(path: str, workload: str):
+    """HBM bytes per fused-kernel launch from a profile summary ('key: value'
+    lines), if it was measured on this workload."""
+    p = Path(path)
+    if not p.exists():
+        return None
+    kv = {}
+    for line in p.read_text().splitlines():
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.strip()
+    if kv.get("workload") != workload or "hbm_bytes_per_launch" not in kv:
+        return None
+    return int(float(kv["hbm_bytes_per_launch"]))
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -50,8 +66,18 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events")
-    ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes per launch")
+    ap.add_argument("--traffic", default=str(ROOT / "profiles" / "g2p2g_channel256_traffic.txt"),
+                    help="rocprofv3-derived HBM bytes per launch of the fused kernel (key: value text)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run the calibration copy of BYTES and print its bandwidth")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        mpm = importlib.import_module("water-sim-cuda_amd")
+        ms = mpm.probe_copy(args.copy_probe, 20)
+        print(json.dumps({"copy_bytes": args.copy_probe, "ms_per_copy": ms,
+                          "GBps_read_plus_write": 2 * args.copy_probe / ms / 1e6}), flush=True)
+        return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the engine is single-GPU (multi-GPU slab "
@@ -104,9 +130,7 @@ def main() -> int:
         avg_s = ms / cnt / 1e3
         bytes_launch = BYTES_PER_PARTICLE * n + BYTES_PER_ACTIVE_NODE * active
         achieved = bytes_launch / avg_s / 1e9
-        traffic = None
-        if args.traffic and Path(args.traffic).exists():
-            traffic = json.loads(Path(args.traffic).read_text()).get("hbm_bytes_per_launch")
+        traffic = read_traffic(args.traffic, sc["name"])
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_transfer<G2P2G>", "avg_launch_ms": round(ms / cnt, 5),

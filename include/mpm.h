@@ -189,6 +189,14 @@ int mpm_engine_debug_grid_update(mpm_engine* h, const float* grid4_in, float* gr
  * mass/velocity grid; updates the particle state in place. */
 int mpm_engine_debug_g2p(mpm_engine* h, const float* grid4_in);
 
+/* ---- measurement aid (no reference counterpart) ---- */
+
+/* Copies `bytes` (rounded down to 16) between two fresh device buffers
+ * `iters` times with a streaming kernel; *ms_per_copy receives the mean time
+ * of one copy.  Used to calibrate HBM byte counters and to report the
+ * achievable copy bandwidth next to the engine's figures. */
+int mpm_probe_copy(size_t bytes, int iters, double* ms_per_copy);
+
 /* ---- scene generators (build components; no reference counterpart) ---- */
 
 /* Jittered particles, 8 per cell (one per 2x2x2 sub-cell, uniform jitter from

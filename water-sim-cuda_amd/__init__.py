@@ -137,6 +137,7 @@ def load_library() -> ctypes.CDLL:
         "mpm_engine_debug_p2g": (ctypes.c_int, [eng, c_float_p]),
         "mpm_engine_debug_grid_update": (ctypes.c_int, [eng, c_float_p, c_float_p]),
         "mpm_engine_debug_g2p": (ctypes.c_int, [eng, c_float_p]),
+        "mpm_probe_copy": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, c_dbl_p]),
         "mpm_scene_fill": (
             ctypes.c_int,
             [ctypes.c_int, c_float_p, c_float_p, ctypes.c_uint64, c_float_p, ctypes.c_int64, c_i64_p],
@@ -388,6 +389,13 @@ class Engine:
     def debug_g2p(self, grid4) -> None:
         g = _f32(grid4).reshape(4, -1)
         _check(self._lib.mpm_engine_debug_g2p(self._h, _fptr(g)))
+
+
+def probe_copy(nbytes: int, iters: int = 20) -> float:
+    """Mean milliseconds of one device-to-device streaming copy of nbytes."""
+    ms = ctypes.c_double(0.0)
+    _check(load_library().mpm_probe_copy(int(nbytes), int(iters), ctypes.byref(ms)))
+    return ms.value
 
 
 def gpu_available() -> bool:

@@ -31,11 +31,15 @@ enum : int {
     A_STATE = 24
 };
 
-// Chunk = up to CHUNK particles of one 4^3-cell Morton block, processed by one
-// transfer workgroup.  Particle of in-chunk rank q lives in slot
-// chunk*CHUNK + (q % S)*64 + q / S, S = ceil(count/64): a wave's 64 lanes hold
-// particles S ranks apart, so they mostly sit in different cells and update
-// different arena nodes at the same stencil offset.
+// Chunk = up to CHUNK particles whose base cells (trunc(x - 0.5), the lowest
+// node of the quadratic stencil) lie in one 4^3 Morton block, processed by one
+// transfer wave.  The engine orders particles by the Morton key of the base
+// cell, so particles sharing a base node have consecutive ranks.  Particle of
+// in-chunk rank q lives in slot chunk*CHUNK + (q % S)*64 + q / S,
+// S = ceil(count/64): consecutive ranks land in one lane in successive rounds,
+// so the 64 lanes of a round hold particles with distinct base nodes (as long
+// as no base node has more than S particles) and update distinct arena nodes
+// at each stencil offset.
 struct ChunkDesc {
     int start;       // first index of this chunk in the sorted order
     int count;       // particles in the chunk
@@ -43,14 +47,13 @@ struct ChunkDesc {
     int pad0, pad1, pad2;
 };
 
-constexpr int BLOCK_CELLS = 4;                             // chunk block edge, cells
-constexpr int BLOCK_SHIFT = 6;                             // log2(4^3): Morton key -> block
-// Drift allowance of the tiles, in nodes.  A particle's base node leaves its
-// block's range only after it moves half a cell out of the block, so 0 keeps
-// all particles in their tiles between re-sorts unless they move that far.
+constexpr int BLOCK_CELLS = 4;  // chunk block edge, base cells
+constexpr int BLOCK_SHIFT = 6;  // log2(4^3): Morton key -> block
+// Drift allowance of the tiles, in nodes.  Particles whose base node moved
+// out of their block since the last re-sort take the global-memory path.
 constexpr int TILE_MARGIN = 0;
-constexpr int TILE_D = BLOCK_CELLS + 3 + 2 * TILE_MARGIN;  // 7 nodes per axis
-constexpr int TILE_N = TILE_D * TILE_D * TILE_D;           // 343
+constexpr int TILE_D = BLOCK_CELLS + 2 + 2 * TILE_MARGIN;  // 6 nodes per axis
+constexpr int TILE_N = TILE_D * TILE_D * TILE_D;           // 216
 constexpr int TRANSFER_THREADS = 64;  // one wave per chunk (wave-private LDS arena)
 
 // LDS slot pitches of the tile/arena (16-byte slots).  Lanes of one LDS access
@@ -61,7 +64,7 @@ constexpr int TRANSFER_THREADS = 64;  // one wave per chunk (wave-private LDS ar
 #define MPM_TILE_PY 10
 #endif
 #ifndef MPM_TILE_PZ
-#define MPM_TILE_PZ 76
+#define MPM_TILE_PZ 60
 #endif
 constexpr int TILE_PY = MPM_TILE_PY;
 constexpr int TILE_PZ = MPM_TILE_PZ;
@@ -120,6 +123,14 @@ __device__ __forceinline__ uint32_t key_coord(float p) {
 __device__ __forceinline__ uint32_t sort_key(float x, float y, float z) {
     uint32_t k = (expand_bits(key_coord(z)) << 2) | (expand_bits(key_coord(y)) << 1) | expand_bits(key_coord(x));
     return k ^ 0x80000000u;
+}
+// Engine-internal ordering key: Morton code of the base cell, each coordinate
+// clamped to [0, 1023] (particles outside the grid only need some block).
+__device__ __forceinline__ uint32_t base_key(float x, float y, float z) {
+    const uint32_t bx = (uint32_t)min(max(base_of(x), 0), 1023);
+    const uint32_t by = (uint32_t)min(max(base_of(y), 0), 1023);
+    const uint32_t bz = (uint32_t)min(max(base_of(z), 0), 1023);
+    return (expand_bits(bz) << 2) | (expand_bits(by) << 1) | expand_bits(bx);
 }
 // Inverse of expand_bits on the 10 low interleaved bits.
 __device__ __forceinline__ uint32_t compact_bits(uint32_t v) {

@@ -427,7 +427,7 @@ int mpm_engine_init_particles(mpm_engine* h, int n, const float* soa26) {
     if ((rc = ensure_sort_scratch(h, (size_t)n))) { dev_free(tmp); return rc; }
     h->chunk_limit = 2 * (n / h->chunk_cap) + 1024;
     if ((rc = dev_alloc(&h->chunks, (size_t)h->chunk_limit))) { dev_free(tmp); return rc; }
-    launch_keys_linear(tmp, tmp + (size_t)n, tmp + 2 * (size_t)n, n, h->sc.keys[0], h->sc.vals[0], h->stream);
+    launch_keys_linear(tmp, tmp + (size_t)n, tmp + 2 * (size_t)n, n, true, h->sc.keys[0], h->sc.vals[0], h->stream);
     h->pcur = 1;  // sort_and_reorder writes into buffer 0
     rc = sort_and_reorder(h, tmp, (size_t)n, nullptr, MASK_ALL);
     hipStreamSynchronize(h->stream);
@@ -569,7 +569,7 @@ int mpm_engine_get_sort(mpm_engine* h, int32_t* keys_out, int32_t* perm_out) {
     if ((rc = dev_alloc(&tmp, (size_t)3 * n))) return rc;
     launch_export_positions(h->chunks, h->nchunks, h->chunk_cap, h->planes[h->pcur], h->plane_cap[h->pcur],
                             h->orig[h->pcur], n, tmp, h->stream);
-    launch_keys_linear(tmp, tmp + n, tmp + 2 * (size_t)n, n, h->sc.keys[0], h->sc.vals[0], h->stream);
+    launch_keys_linear(tmp, tmp + n, tmp + 2 * (size_t)n, n, false, h->sc.keys[0], h->sc.vals[0], h->stream);
     const int r = radix_sort_pairs(h->sc, n, h->stream);
     hipError_t e1 = hipMemcpyAsync(keys_out, h->sc.keys[r], sizeof(int32_t) * n, hipMemcpyDeviceToHost, h->stream);
     hipError_t e2 = hipMemcpyAsync(perm_out, h->sc.vals[r], sizeof(int32_t) * n, hipMemcpyDeviceToHost, h->stream);
@@ -673,6 +673,40 @@ int mpm_engine_debug_g2p(mpm_engine* h, const float* in) {
     if (h->n > 0) run_transfer(h, XFER_G2P);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
+    return MPM_OK;
+}
+
+int mpm_probe_copy(size_t bytes, int iters, double* ms_per_copy) {
+    if (bytes < 16 || iters < 1 || !ms_per_copy) return fail(MPM_EINVAL, "bad copy probe arguments");
+    const size_t n = bytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    int rc;
+    if ((rc = dev_alloc(&a, n)) || (rc = dev_alloc(&b, n))) {
+        dev_free(a);
+        return rc;
+    }
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t err = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+    if (err == hipSuccess) err = hipMemsetAsync(a, 0, n * 16, s);
+    if (err == hipSuccess) {
+        launch_copy(a, b, n, s);  // warm-up
+        hipEventRecord(e0, s);
+        for (int i = 0; i < iters; ++i) launch_copy(a, b, n, s);
+        hipEventRecord(e1, s);
+        err = hipEventSynchronize(e1);
+    }
+    float ms = 0.0f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (s) hipStreamDestroy(s);
+    dev_free(a);
+    dev_free(b);
+    if (err != hipSuccess) return fail(MPM_EHIP, std::string("copy probe: ") + hipGetErrorString(err));
+    *ms_per_copy = ms / iters;
     return MPM_OK;
 }
 

@@ -44,6 +44,7 @@ void launch_grid_update(const DevCfg& c, GridBuf m, GridBuf v_clear, bool clear_
 
 void launch_make_sdf(const DevCfg& c, const float* tris, int nt, float* sdf, hipStream_t s);
 void launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
+void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s);
 void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t s);
 void launch_init_F(float* soa26, int n, hipStream_t s);
 
@@ -71,13 +72,14 @@ struct SortScratch {
 size_t radix_hist_elems(size_t n);
 size_t scan_partial_elems(size_t n);
 
-// Keys of the particles in the current slot layout, written densely at
-// chunk.start + rank (vals = slot).
+// Base-cell keys of the particles in the current slot layout, written densely
+// at chunk.start + rank (vals = slot).
 void launch_keys_slots(const ChunkDesc* ch, int nchunks, int chunk_cap, const float* part, size_t stride,
                        uint32_t* keys, uint32_t* vals, hipStream_t s);
-// Keys of caller-ordered position planes (vals = index).
-void launch_keys_linear(const float* px, const float* py, const float* pz, int n, uint32_t* keys, uint32_t* vals,
-                        hipStream_t s);
+// Keys of caller-ordered position planes (vals = index): the engine's
+// base-cell keys (internal) or the reference's setKeys keys.
+void launch_keys_linear(const float* px, const float* py, const float* pz, int n, bool internal, uint32_t* keys,
+                        uint32_t* vals, hipStream_t s);
 // Stable LSD radix sort of (keys, vals) pairs; result in keys[r], vals[r] with
 // r returned (0 or 1).  Passes over key bytes on which all keys agree are
 // skipped (a stable sort by a constant digit is the identity).
@@ -85,8 +87,8 @@ int radix_sort_pairs(SortScratch& sc, int n, hipStream_t s);
 // Exclusive scan of n uint32 values; total written to *total_dev.
 void scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* partial, uint32_t* total_dev,
                         hipStream_t s);
-// Splits the sorted keys into chunks (one 4^3 Morton block per chunk, at most
-// chunk_cap particles).  Writes up to chunk_limit descriptors; the chunk count
+// Splits the sorted base-cell keys into chunks (one 4^3 Morton block per
+// chunk, at most chunk_cap particles).  Writes up to chunk_limit descriptors; the chunk count
 // (possibly above the limit) lands in sc.counters[3].
 void build_chunks(SortScratch& sc, int r, int n, int chunk_cap, ChunkDesc* chunks, int chunk_limit, hipStream_t s);
 // Gathers the particles into the chunked slot layout.  attr_mask selects the

@@ -184,7 +184,19 @@ __global__ __launch_bounds__(256) void k_export_positions(const ChunkDesc* __res
     }
 }
 
+// Streaming copy (measurement aid: calibrates HBM byte counters and gives the
+// achievable copy bandwidth).
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
+
+void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_copy, dim3(256 * 8 * 4), dim3(256), 0, s, src, dst, n);
+}
 
 void launch_grid_update(const DevCfg& c, GridBuf m, GridBuf v_clear, bool clear_v, const float* sdf, int nblocks,
                         hipStream_t s) {

@@ -217,17 +217,17 @@ __global__ __launch_bounds__(256) void k_keys_slots(const ChunkDesc* __restrict_
         const int q = lane * S + j;
         if (q >= d.count) continue;
         const size_t slot = (size_t)blockIdx.x * chunk_cap + (size_t)j * 64 + lane;
-        keys[d.start + q] = sort_key(part[A_PX * stride + slot], part[A_PY * stride + slot], part[A_PZ * stride + slot]);
+        keys[d.start + q] = base_key(part[A_PX * stride + slot], part[A_PY * stride + slot], part[A_PZ * stride + slot]);
         vals[d.start + q] = (uint32_t)slot;
     }
 }
 
 __global__ void k_keys_linear(const float* __restrict__ px, const float* __restrict__ py,
-                              const float* __restrict__ pz, int n, uint32_t* __restrict__ keys,
+                              const float* __restrict__ pz, int n, int internal, uint32_t* __restrict__ keys,
                               uint32_t* __restrict__ vals) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    keys[i] = sort_key(px[i], py[i], pz[i]);
+    keys[i] = internal ? base_key(px[i], py[i], pz[i]) : sort_key(px[i], py[i], pz[i]);
     vals[i] = (uint32_t)i;
 }
 
@@ -265,10 +265,10 @@ __global__ void k_chunk_desc(const int* __restrict__ seg_start, const uint32_t* 
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n || s >= (int)counters[2]) return;
     const int st = seg_start[s], len = seg_start[s + 1] - st;
-    const uint32_t k = keys[st] ^ 0x80000000u;
+    const uint32_t k = keys[st];  // base_key: Morton code of the base cell
     const int cx = (int)(compact_bits(k) & ~3u), cy = (int)(compact_bits(k >> 1) & ~3u),
               cz = (int)(compact_bits(k >> 2) & ~3u);
-    const int o = 1 + TILE_MARGIN;
+    const int o = TILE_MARGIN;
     int c = (int)first[s];
     for (int off = 0; off < len; off += cap, ++c) {
         if (c >= limit) return;
@@ -329,10 +329,11 @@ void launch_keys_slots(const ChunkDesc* ch, int nchunks, int chunk_cap, const fl
     hipLaunchKernelGGL(k_keys_slots, dim3(nchunks), dim3(256), 0, s, ch, chunk_cap, part, stride, keys, vals);
 }
 
-void launch_keys_linear(const float* px, const float* py, const float* pz, int n, uint32_t* keys, uint32_t* vals,
-                        hipStream_t s) {
+void launch_keys_linear(const float* px, const float* py, const float* pz, int n, bool internal, uint32_t* keys,
+                        uint32_t* vals, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_keys_linear, dim3((n + 255) / 256), dim3(256), 0, s, px, py, pz, n, keys, vals);
+    hipLaunchKernelGGL(k_keys_linear, dim3((n + 255) / 256), dim3(256), 0, s, px, py, pz, n, internal ? 1 : 0, keys,
+                       vals);
 }
 
 int radix_sort_pairs(SortScratch& sc, int n, hipStream_t s) {
