@@ -170,9 +170,11 @@ int mpm_engine_get_grid(mpm_engine* h, float* grid4_out);
  * Resets the accumulators. */
 int mpm_engine_get_stage_times(mpm_engine* h, double* out, int nstages);
 
-/* Diagnostic counters of the last substep: out[0] = chunks, out[1] =
- * particles whose stencil left their workgroup's LDS tile (global-atomic
- * path), out[2] = active 4^3 grid blocks. */
+/* Diagnostic counters of the last mpm_engine_steps call: out[0] = chunks,
+ * out[1] = particle-substeps whose stencil had left their chunk's LDS tile
+ * (global-memory path), out[2] = active 4^3 grid blocks, out[3] =
+ * particle-substeps that shared a base node with another lane of their round
+ * (global-atomic path). */
 int mpm_engine_get_counters(mpm_engine* h, int64_t* out, int n);
 
 /* ---- stage access for parity tests (no reference counterpart) ---- */

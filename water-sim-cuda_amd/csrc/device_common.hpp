@@ -31,20 +31,21 @@ enum : int {
     A_STATE = 24
 };
 
-// Chunk = up to CHUNK particles whose base cells (trunc(x - 0.5), the lowest
-// node of the quadratic stencil) lie in one 4^3 Morton block, processed by one
-// transfer wave.  The engine orders particles by the Morton key of the base
-// cell, so particles sharing a base node have consecutive ranks.  Particle of
-// in-chunk rank q lives in slot chunk*CHUNK + (q % S)*64 + q / S,
-// S = ceil(count/64): consecutive ranks land in one lane in successive rounds,
-// so the 64 lanes of a round hold particles with distinct base nodes (as long
-// as no base node has more than S particles) and update distinct arena nodes
-// at each stencil offset.
+// Chunk = up to CHUNK consecutive particles (in base-cell Morton order) whose
+// base cells (trunc(x - 0.5), the lowest node of the quadratic stencil) lie in
+// one 4^3 Morton block, processed by one transfer wave.  A chunk therefore
+// holds at most 64 groups of particles sharing a base cell.  It occupies the
+// slots [start, start + count) in rounds: round r holds the r-th member of
+// every group with more than r members (groups in key order), n_r <= 64
+// particles stored contiguously, with n_r in rounds[start + r].  The lanes of
+// a round hence hold particles with distinct base nodes and update distinct
+// arena nodes at each stencil offset.
 struct ChunkDesc {
-    int start;       // first index of this chunk in the sorted order
+    int start;       // first slot (= first index of the chunk in sorted order)
     int count;       // particles in the chunk
     int ox, oy, oz;  // node origin of the chunk's LDS tile
-    int pad0, pad1, pad2;
+    int S;           // rounds (largest group)
+    int pad0, pad1;
 };
 
 constexpr int BLOCK_CELLS = 4;  // chunk block edge, base cells
@@ -124,12 +125,13 @@ __device__ __forceinline__ uint32_t sort_key(float x, float y, float z) {
     uint32_t k = (expand_bits(key_coord(z)) << 2) | (expand_bits(key_coord(y)) << 1) | expand_bits(key_coord(x));
     return k ^ 0x80000000u;
 }
-// Engine-internal ordering key: Morton code of the base cell, each coordinate
-// clamped to [0, 1023] (particles outside the grid only need some block).
-__device__ __forceinline__ uint32_t base_key(float x, float y, float z) {
-    const uint32_t bx = (uint32_t)min(max(base_of(x), 0), 1023);
-    const uint32_t by = (uint32_t)min(max(base_of(y), 0), 1023);
-    const uint32_t bz = (uint32_t)min(max(base_of(z), 0), 1023);
+// Engine-internal ordering key: Morton code of the base cell with each
+// coordinate clamped into the grid (particles outside the grid only need some
+// block), so keys have 3 * ceil(log2(max SIZE)) bits.
+__device__ __forceinline__ uint32_t base_key(const DevCfg& c, float x, float y, float z) {
+    const uint32_t bx = (uint32_t)min(max(base_of(x), 0), c.sx - 1);
+    const uint32_t by = (uint32_t)min(max(base_of(y), 0), c.sy - 1);
+    const uint32_t bz = (uint32_t)min(max(base_of(z), 0), c.sz - 1);
     return (expand_bits(bz) << 2) | (expand_bits(by) << 1) | expand_bits(bx);
 }
 // Inverse of expand_bits on the 10 low interleaved bits.

@@ -68,8 +68,10 @@ struct mpm_engine {
     int pcur = 0;
 
     ChunkDesc* chunks = nullptr;
-    int chunk_limit = 0;
-    int nchunks = 0;
+    uint8_t* rounds = nullptr;  // particles per round of each chunk, at chunk.start + round
+    int chunk_limit = 0;  // bound on the chunk count (min(n, 4^3 blocks of the grid) + n / CHUNK + 1)
+    int chunk_est = 0;    // chunk count at the last host synchronisation (sizes launches)
+    int key_bits = 0;     // bits of the engine's base-cell keys
 
     GridBuf grid[2] = {};
     int vcur = 0;  // grid[vcur]: velocities of the last update; grid[1 - vcur]: accumulates
@@ -82,6 +84,7 @@ struct mpm_engine {
     int64_t substeps = 0;
     int64_t sorted_at = 0;
     int64_t last_overflow = 0;
+    int64_t last_claim_losers = 0;
 
     struct Ev {
         hipEvent_t a, b;
@@ -145,6 +148,7 @@ void free_particles(mpm_engine* h) {
         h->plane_cap[i] = 0;
     }
     dev_free(h->chunks);
+    dev_free(h->rounds);
     dev_free(h->sc.keys[0]);
     dev_free(h->sc.keys[1]);
     dev_free(h->sc.vals[0]);
@@ -156,7 +160,7 @@ void free_particles(mpm_engine* h) {
     dev_free(h->sc.seg_nch);
     h->sc.cap = h->sc.hist_cap = h->sc.partial_cap = 0;
     h->chunk_limit = 0;
-    h->nchunks = 0;
+    h->chunk_est = 0;
     h->have_particles = false;
     h->n = 0;
 }
@@ -197,31 +201,17 @@ int read_counter(mpm_engine* h, uint32_t* dev, uint32_t* out) {
     return MPM_OK;
 }
 
-// Sorts sc.keys[0]/vals[0] (n entries), builds the chunk table and gathers the
-// particles from (src, src_stride, src_orig) into the alternate plane buffer.
+// Sorts sc.keys[0]/vals[0] (n base-cell keys), builds the chunk table and
+// gathers the particles from (src, src_stride, src_orig) into the alternate
+// plane buffer.  Nothing here waits for the device: the chunk count stays in
+// device memory and is bounded by chunk_limit by construction.
 int sort_and_reorder(mpm_engine* h, const float* src, size_t src_stride, const int* src_orig, uint32_t mask) {
     const int n = h->n;
-    const int r = radix_sort_pairs(h->sc, n, h->stream);
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        build_chunks(h->sc, r, n, h->chunk_cap, h->chunks, h->chunk_limit, h->stream);
-        uint32_t nch = 0;
-        int rc = read_counter(h, h->sc.counters + 3, &nch);
-        if (rc) return rc;
-        if ((int)nch <= h->chunk_limit) {
-            h->nchunks = (int)nch;
-            break;
-        }
-        dev_free(h->chunks);
-        h->chunk_limit = (int)nch + nch / 4 + 64;
-        if ((rc = dev_alloc(&h->chunks, (size_t)h->chunk_limit))) return rc;
-    }
+    const int r = radix_sort_pairs(h->sc, n, h->key_bits, h->stream);
+    build_chunks(h->sc, r, n, h->chunk_cap, h->chunks, h->chunk_limit, h->stream);
     const int dst = 1 - h->pcur;
-    if ((uint64_t)h->chunk_limit * (uint64_t)h->chunk_cap >= (1ull << 32))
-        return fail(MPM_ELIMIT, "particle slots exceed 2^32 (too many sparsely filled 4^3-cell blocks)");
-    int rc = ensure_planes(h, dst, (size_t)h->chunk_limit * h->chunk_cap);
-    if (rc) return rc;
-    launch_reorder(h->chunks, h->nchunks, h->chunk_cap, h->sc.vals[r], src, src_stride, src_orig, h->planes[dst],
-                   h->plane_cap[dst], h->orig[dst], mask, h->stream);
+    launch_reorder(h->chunks, h->sc.counters + 3, h->chunk_est, h->sc.keys[r], h->sc.vals[r], src, src_stride,
+                   src_orig, h->planes[dst], h->plane_cap[dst], h->orig[dst], h->rounds, mask, h->stream);
     HIPCHK(hipGetLastError());
     h->pcur = dst;
     return MPM_OK;
@@ -232,8 +222,10 @@ constexpr uint32_t MASK_POS_F = 0x7u | (0x1FFu << A_F0);
 
 int resort(mpm_engine* h, bool full) {
     StageTimer t(h, ST_SORT);
-    launch_keys_slots(h->chunks, h->nchunks, h->chunk_cap, h->planes[h->pcur], h->plane_cap[h->pcur], h->sc.keys[0],
-                      h->sc.vals[0], h->stream);
+    const float* pl = h->planes[h->pcur];
+    const size_t st = h->plane_cap[h->pcur];
+    launch_keys_linear(pl + A_PX * st, pl + A_PY * st, pl + A_PZ * st, h->n, true, h->dc, h->sc.keys[0],
+                       h->sc.vals[0], h->stream);
     return sort_and_reorder(h, h->planes[h->pcur], h->plane_cap[h->pcur], h->orig[h->pcur], full ? MASK_ALL : MASK_POS_F);
 }
 
@@ -243,9 +235,8 @@ TransferArgs make_args(mpm_engine* h) {
     a.part = h->planes[h->pcur];
     a.stride = h->plane_cap[h->pcur];
     a.chunks = h->chunks;
-    a.nchunks = h->nchunks;
-    a.chunk_cap = h->chunk_cap;
-    a.per_xcd = (h->nchunks + 7) / 8;
+    a.nchunks = h->sc.counters + 3;
+    a.rounds = h->rounds;
     const GridBuf& V = h->grid[h->vcur];
     const GridBuf& M = h->grid[1 - h->vcur];
     a.vel[0] = V.plane[1];
@@ -261,7 +252,7 @@ TransferArgs make_args(mpm_engine* h) {
 void run_transfer(mpm_engine* h, int mode) {
     const int stage = mode == XFER_P2G ? ST_P2G : (mode == XFER_G2P ? ST_G2P : ST_G2P2G);
     StageTimer t(h, stage);
-    launch_transfer(mode, make_args(h), h->stream);
+    launch_transfer(mode, make_args(h), h->chunk_est, h->stream);
 }
 
 void run_grid_update(mpm_engine* h) {
@@ -322,6 +313,11 @@ int mpm_engine_create(const mpm_sim_config* cfg_in, mpm_engine** out) {
     d.fbx = (d.sx + 3) / 4; d.fby = (d.sy + 3) / 4; d.fbz = (d.sz + 3) / 4;
     h->G = (size_t)G;
     h->nfb = d.fbx * d.fby * d.fbz;
+    {
+        int maxdim = std::max(d.sx, std::max(d.sy, d.sz)), bits = 0;
+        while ((1 << bits) < maxdim) ++bits;
+        h->key_bits = 3 * std::max(bits, 1);
+    }
     auto cleanup = [&](int rc) {
         mpm_engine_destroy(h);
         return rc;
@@ -340,8 +336,9 @@ int mpm_engine_create(const mpm_sim_config* cfg_in, mpm_engine** out) {
     if ((rc = dev_alloc(&h->sdf, h->G))) return cleanup(rc);
     launch_fill_f32(h->sdf, h->G, 1e15f, h->stream);  // makeSDF with no triangles
     if ((rc = dev_alloc(&h->sc.counters, 64))) return cleanup(rc);
-    if ((rc = dev_alloc(&h->overflow, 1))) return cleanup(rc);
-    if (hipMemsetAsync(h->overflow, 0, sizeof(uint32_t), h->stream) != hipSuccess) return cleanup(fail(MPM_EHIP, "memset"));
+    if ((rc = dev_alloc(&h->overflow, 2))) return cleanup(rc);
+    if (hipMemsetAsync(h->overflow, 0, 2 * sizeof(uint32_t), h->stream) != hipSuccess)
+        return cleanup(fail(MPM_EHIP, "memset"));
     if (hipHostMalloc((void**)&h->pinned, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
         return cleanup(fail(MPM_ENOMEM, "hipHostMalloc failed"));
     if (hipStreamSynchronize(h->stream) != hipSuccess) return cleanup(fail(MPM_EHIP, "stream sync"));
@@ -424,17 +421,26 @@ int mpm_engine_init_particles(mpm_engine* h, int n, const float* soa26) {
     if ((rc = dev_alloc(&tmp, (size_t)MPM_PARTICLE_ATTRS * n))) return rc;
     HIPCHK(hipMemcpyAsync(tmp, soa26, sizeof(float) * MPM_PARTICLE_ATTRS * (size_t)n, hipMemcpyHostToDevice, h->stream));
     launch_init_F(tmp, n, h->stream);
-    if ((rc = ensure_sort_scratch(h, (size_t)n))) { dev_free(tmp); return rc; }
-    h->chunk_limit = 2 * (n / h->chunk_cap) + 1024;
-    if ((rc = dev_alloc(&h->chunks, (size_t)h->chunk_limit))) { dev_free(tmp); return rc; }
-    launch_keys_linear(tmp, tmp + (size_t)n, tmp + 2 * (size_t)n, n, true, h->sc.keys[0], h->sc.vals[0], h->stream);
+    const int64_t blocks = (int64_t)h->dc.fbx * h->dc.fby * h->dc.fbz;
+    h->chunk_limit = (int)(std::min<int64_t>(n, blocks) + (n + h->chunk_cap - 1) / h->chunk_cap + 1);
+    h->chunk_est = h->chunk_limit;
+    if ((rc = ensure_sort_scratch(h, (size_t)n)) || (rc = dev_alloc(&h->chunks, (size_t)h->chunk_limit)) ||
+        (rc = dev_alloc(&h->rounds, (size_t)n)) ||
+        (rc = ensure_planes(h, 0, (size_t)n)) || (rc = ensure_planes(h, 1, (size_t)n))) {
+        dev_free(tmp);
+        return rc;
+    }
+    launch_keys_linear(tmp, tmp + (size_t)n, tmp + 2 * (size_t)n, n, true, h->dc, h->sc.keys[0], h->sc.vals[0],
+                       h->stream);
     h->pcur = 1;  // sort_and_reorder writes into buffer 0
     rc = sort_and_reorder(h, tmp, (size_t)n, nullptr, MASK_ALL);
     hipStreamSynchronize(h->stream);
     dev_free(tmp);
     if (rc) return rc;
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(h->stream));
+    uint32_t nch = 0;
+    if ((rc = read_counter(h, h->sc.counters + 3, &nch))) return rc;
+    h->chunk_est = (int)nch;
     h->have_particles = true;
     return MPM_OK;
 }
@@ -480,7 +486,7 @@ int mpm_engine_steps(mpm_engine* h, int k) {
         h->substeps += k;
         return MPM_OK;
     }
-    HIPCHK(hipMemsetAsync(h->overflow, 0, sizeof(uint32_t), h->stream));
+    HIPCHK(hipMemsetAsync(h->overflow, 0, 2 * sizeof(uint32_t), h->stream));
     for (int s = 0; s < k; ++s) {
         if (s == 0) {
             if (h->substeps - h->sorted_at >= h->resort_every) {
@@ -503,9 +509,13 @@ int mpm_engine_steps(mpm_engine* h, int k) {
         }
     }
     HIPCHK(hipGetLastError());
-    uint32_t ov = 0;
-    if ((rc = read_counter(h, h->overflow, &ov))) return rc;
+    uint32_t ov = 0, cl = 0, nch = 0;
+    if ((rc = read_counter(h, h->overflow, &ov)) || (rc = read_counter(h, h->overflow + 1, &cl)) ||
+        (rc = read_counter(h, h->sc.counters + 3, &nch)))
+        return rc;
     h->last_overflow = ov;
+    h->last_claim_losers = cl;
+    h->chunk_est = (int)nch;
     return MPM_OK;
 }
 
@@ -527,8 +537,8 @@ int mpm_engine_get_particles(mpm_engine* h, float* out) {
     if (h->n == 0) return MPM_OK;
     float* tmp = nullptr;
     if ((rc = dev_alloc(&tmp, (size_t)MPM_PARTICLE_ATTRS * h->n))) return rc;
-    launch_export_soa26(h->chunks, h->nchunks, h->chunk_cap, h->planes[h->pcur], h->plane_cap[h->pcur], h->orig[h->pcur],
-                        h->n, h->dc.vol, h->dc.mass, tmp, h->stream);
+    launch_export_soa26(h->planes[h->pcur], h->plane_cap[h->pcur], h->orig[h->pcur], h->n, h->dc.vol, h->dc.mass, tmp,
+                        h->stream);
     hipError_t e = hipMemcpyAsync(out, tmp, sizeof(float) * MPM_PARTICLE_ATTRS * (size_t)h->n, hipMemcpyDeviceToHost,
                                   h->stream);
     hipError_t e2 = hipStreamSynchronize(h->stream);
@@ -544,8 +554,7 @@ int mpm_engine_get_positions(mpm_engine* h, float* xyz) {
     if (h->n == 0) return MPM_OK;
     float* tmp = nullptr;
     if ((rc = dev_alloc(&tmp, (size_t)3 * h->n))) return rc;
-    launch_export_positions(h->chunks, h->nchunks, h->chunk_cap, h->planes[h->pcur], h->plane_cap[h->pcur],
-                            h->orig[h->pcur], h->n, tmp, h->stream);
+    launch_export_positions(h->planes[h->pcur], h->plane_cap[h->pcur], h->orig[h->pcur], h->n, tmp, h->stream);
     std::vector<float> planes((size_t)3 * h->n);
     hipError_t e = hipMemcpyAsync(planes.data(), tmp, sizeof(float) * 3 * (size_t)h->n, hipMemcpyDeviceToHost, h->stream);
     hipError_t e2 = hipStreamSynchronize(h->stream);
@@ -567,10 +576,9 @@ int mpm_engine_get_sort(mpm_engine* h, int32_t* keys_out, int32_t* perm_out) {
     if (n == 0) return MPM_OK;
     float* tmp = nullptr;
     if ((rc = dev_alloc(&tmp, (size_t)3 * n))) return rc;
-    launch_export_positions(h->chunks, h->nchunks, h->chunk_cap, h->planes[h->pcur], h->plane_cap[h->pcur],
-                            h->orig[h->pcur], n, tmp, h->stream);
-    launch_keys_linear(tmp, tmp + n, tmp + 2 * (size_t)n, n, false, h->sc.keys[0], h->sc.vals[0], h->stream);
-    const int r = radix_sort_pairs(h->sc, n, h->stream);
+    launch_export_positions(h->planes[h->pcur], h->plane_cap[h->pcur], h->orig[h->pcur], n, tmp, h->stream);
+    launch_keys_linear(tmp, tmp + n, tmp + 2 * (size_t)n, n, false, h->dc, h->sc.keys[0], h->sc.vals[0], h->stream);
+    const int r = radix_sort_pairs(h->sc, n, 0, h->stream);
     hipError_t e1 = hipMemcpyAsync(keys_out, h->sc.keys[r], sizeof(int32_t) * n, hipMemcpyDeviceToHost, h->stream);
     hipError_t e2 = hipMemcpyAsync(perm_out, h->sc.vals[r], sizeof(int32_t) * n, hipMemcpyDeviceToHost, h->stream);
     hipError_t e3 = hipStreamSynchronize(h->stream);
@@ -620,7 +628,7 @@ int mpm_engine_get_stage_times(mpm_engine* h, double* out, int nstages) {
 
 int mpm_engine_get_counters(mpm_engine* h, int64_t* out, int n) {
     if (!h || !out) return fail(MPM_EINVAL, "null argument");
-    int64_t v[3] = {h->nchunks, h->last_overflow, 0};
+    int64_t v[4] = {h->chunk_est, h->last_overflow, 0, h->last_claim_losers};
     if (n > 2) {
         std::vector<uint8_t> f((size_t)h->nfb);
         hipSetDevice(h->device);
@@ -630,7 +638,7 @@ int mpm_engine_get_counters(mpm_engine* h, int64_t* out, int n) {
         for (uint8_t b : f) cnt += b != 0;
         v[2] = cnt;
     }
-    for (int i = 0; i < n && i < 3; ++i) out[i] = v[i];
+    for (int i = 0; i < n && i < 4; ++i) out[i] = v[i];
     return MPM_OK;
 }
 

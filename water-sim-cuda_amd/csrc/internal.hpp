@@ -25,17 +25,18 @@ struct TransferArgs {
     float* part;          // particle planes, attr*stride + slot
     size_t stride;
     const ChunkDesc* chunks;
-    int nchunks;
-    int chunk_cap;        // slots per chunk
-    int per_xcd;          // ceil(nchunks / 8)
+    const uint32_t* nchunks;  // chunk count, on the device
+    const uint8_t* rounds;    // particles per round, at chunk.start + round
     const float* vel[3];  // grid velocity planes read by G2P
     float* acc[4];        // grid mass/momentum planes written by P2G
     uint8_t* acc_flags;
     const float* sdf;
-    unsigned int* overflow;  // particles that took the global-atomic path
+    unsigned int* overflow;  // [0] particles outside their tile, [1] particles that lost a base-node claim
 };
 
-void launch_transfer(int mode, const TransferArgs& a, hipStream_t s);
+// `grid_chunks` sizes the launch (a multiple of 8 workgroups is used); the
+// kernel loops until the device chunk count is covered.
+void launch_transfer(int mode, const TransferArgs& a, int grid_chunks, hipStream_t s);
 
 // Grid update over the 4^3 blocks flagged in `m`, zeroing the blocks flagged in
 // `v_clear` (the previous velocity buffer) and clearing those flags.
@@ -48,11 +49,12 @@ void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s);
 void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t s);
 void launch_init_F(float* soa26, int n, hipStream_t s);
 
-// Export of the slot layout into the caller's order.
-void launch_export_soa26(const ChunkDesc* ch, int nchunks, int chunk_cap, const float* part, size_t stride,
-                         const int* orig, int n, float vol, float mass, float* out26, hipStream_t s);
-void launch_export_positions(const ChunkDesc* ch, int nchunks, int chunk_cap, const float* part, size_t stride,
-                             const int* orig, int n, float* out_planes3, hipStream_t s);
+// Export of the slot layout into the caller's order (slot s holds caller
+// particle orig[s]).
+void launch_export_soa26(const float* part, size_t stride, const int* orig, int n, float vol, float mass, float* out26,
+                         hipStream_t s);
+void launch_export_positions(const float* part, size_t stride, const int* orig, int n, float* out_planes3,
+                             hipStream_t s);
 
 // ---- sorting and chunking ----
 struct SortScratch {
@@ -72,18 +74,16 @@ struct SortScratch {
 size_t radix_hist_elems(size_t n);
 size_t scan_partial_elems(size_t n);
 
-// Base-cell keys of the particles in the current slot layout, written densely
-// at chunk.start + rank (vals = slot).
-void launch_keys_slots(const ChunkDesc* ch, int nchunks, int chunk_cap, const float* part, size_t stride,
-                       uint32_t* keys, uint32_t* vals, hipStream_t s);
-// Keys of caller-ordered position planes (vals = index): the engine's
-// base-cell keys (internal) or the reference's setKeys keys.
-void launch_keys_linear(const float* px, const float* py, const float* pz, int n, bool internal, uint32_t* keys,
-                        uint32_t* vals, hipStream_t s);
+// Keys of position planes (vals = index): the engine's base-cell keys
+// (internal) or the reference's setKeys keys.
+void launch_keys_linear(const float* px, const float* py, const float* pz, int n, bool internal, const DevCfg& c,
+                        uint32_t* keys, uint32_t* vals, hipStream_t s);
 // Stable LSD radix sort of (keys, vals) pairs; result in keys[r], vals[r] with
-// r returned (0 or 1).  Passes over key bytes on which all keys agree are
-// skipped (a stable sort by a constant digit is the identity).
-int radix_sort_pairs(SortScratch& sc, int n, hipStream_t s);
+// r returned (0 or 1).  With key_bits > 0 the keys are known to fit in that
+// many bits and only those digits are sorted; with key_bits = 0 the bits on
+// which all keys agree are found first (one host synchronisation) and their
+// passes skipped (a stable sort by a constant digit is the identity).
+int radix_sort_pairs(SortScratch& sc, int n, int key_bits, hipStream_t s);
 // Exclusive scan of n uint32 values; total written to *total_dev.
 void scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* partial, uint32_t* total_dev,
                         hipStream_t s);
@@ -91,10 +91,12 @@ void scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* p
 // chunk, at most chunk_cap particles).  Writes up to chunk_limit descriptors; the chunk count
 // (possibly above the limit) lands in sc.counters[3].
 void build_chunks(SortScratch& sc, int r, int n, int chunk_cap, ChunkDesc* chunks, int chunk_limit, hipStream_t s);
-// Gathers the particles into the chunked slot layout.  attr_mask selects the
-// planes moved (bit a = attribute a).
-void launch_reorder(const ChunkDesc* ch, int nchunks, int chunk_cap, const uint32_t* vals_sorted, const float* src,
-                    size_t src_stride, const int* src_orig, float* dst, size_t dst_stride, int* dst_orig,
-                    uint32_t attr_mask, hipStream_t s);
+// Gathers the particles into the round layout of their chunks (one wave per
+// chunk; the chunk count is read on the device), writes the round sizes and
+// each chunk's round count.  attr_mask selects the planes moved (bit a =
+// attribute a).
+void launch_reorder(ChunkDesc* ch, const uint32_t* nchunks_dev, int chunk_est, const uint32_t* keys_sorted,
+                    const uint32_t* vals_sorted, const float* src, size_t src_stride, const int* src_orig, float* dst,
+                    size_t dst_stride, int* dst_orig, uint8_t* rounds, uint32_t attr_mask, hipStream_t s);
 
 }  // namespace mpm

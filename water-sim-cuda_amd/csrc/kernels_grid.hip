@@ -1,4 +1,6 @@
 // kernels_grid.hip -- grid update, mesh SDF, state export and small fills.
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace mpm {
@@ -150,37 +152,25 @@ __global__ void k_init_F(float* soa26, int n) {
     soa26[(size_t)A_F8 * n + i] = 1.0f;
 }
 
-__global__ __launch_bounds__(256) void k_export_soa26(const ChunkDesc* __restrict__ ch, int chunk_cap,
-                                                      const float* __restrict__ part, size_t stride,
+__global__ __launch_bounds__(256) void k_export_soa26(const float* __restrict__ part, size_t stride,
                                                       const int* __restrict__ orig, int n, float vol, float mass,
                                                       float* __restrict__ out) {
-    const ChunkDesc d = ch[blockIdx.x];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int S = (d.count + 63) >> 6;
-    for (int j = wave; j < S; j += 4) {
-        if (lane * S + j >= d.count) continue;
-        const size_t slot = (size_t)blockIdx.x * chunk_cap + (size_t)j * 64 + lane;
-        const int id = orig[slot];
-        for (int a = 0; a < A_STATE; ++a) out[(size_t)a * n + id] = part[(size_t)a * stride + slot];
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        const int id = orig[s];
+        for (int a = 0; a < A_STATE; ++a) out[(size_t)a * n + id] = part[(size_t)a * stride + s];
         out[(size_t)24 * n + id] = vol;
         out[(size_t)25 * n + id] = mass;
     }
 }
 
-__global__ __launch_bounds__(256) void k_export_positions(const ChunkDesc* __restrict__ ch, int chunk_cap,
-                                                          const float* __restrict__ part, size_t stride,
+__global__ __launch_bounds__(256) void k_export_positions(const float* __restrict__ part, size_t stride,
                                                           const int* __restrict__ orig, int n,
                                                           float* __restrict__ out) {
-    const ChunkDesc d = ch[blockIdx.x];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int S = (d.count + 63) >> 6;
-    for (int j = wave; j < S; j += 4) {
-        if (lane * S + j >= d.count) continue;
-        const size_t slot = (size_t)blockIdx.x * chunk_cap + (size_t)j * 64 + lane;
-        const int id = orig[slot];
-        out[id] = part[A_PX * stride + slot];
-        out[(size_t)n + id] = part[A_PY * stride + slot];
-        out[(size_t)2 * n + id] = part[A_PZ * stride + slot];
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        const int id = orig[s];
+        out[id] = part[A_PX * stride + s];
+        out[(size_t)n + id] = part[A_PY * stride + s];
+        out[(size_t)2 * n + id] = part[A_PZ * stride + s];
     }
 }
 
@@ -229,18 +219,18 @@ void launch_init_F(float* soa26, int n, hipStream_t s) {
     hipLaunchKernelGGL(k_init_F, dim3((n + 255) / 256), dim3(256), 0, s, soa26, n);
 }
 
-void launch_export_soa26(const ChunkDesc* ch, int nchunks, int chunk_cap, const float* part, size_t stride,
-                         const int* orig, int n, float vol, float mass, float* out26, hipStream_t s) {
-    if (nchunks <= 0) return;
-    hipLaunchKernelGGL(k_export_soa26, dim3(nchunks), dim3(256), 0, s, ch, chunk_cap, part, stride, orig, n, vol, mass,
-                       out26);
+void launch_export_soa26(const float* part, size_t stride, const int* orig, int n, float vol, float mass, float* out26,
+                         hipStream_t s) {
+    if (n <= 0) return;
+    const int blocks = std::min((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_export_soa26, dim3(blocks), dim3(256), 0, s, part, stride, orig, n, vol, mass, out26);
 }
 
-void launch_export_positions(const ChunkDesc* ch, int nchunks, int chunk_cap, const float* part, size_t stride,
-                             const int* orig, int n, float* out_planes3, hipStream_t s) {
-    if (nchunks <= 0) return;
-    hipLaunchKernelGGL(k_export_positions, dim3(nchunks), dim3(256), 0, s, ch, chunk_cap, part, stride, orig, n,
-                       out_planes3);
+void launch_export_positions(const float* part, size_t stride, const int* orig, int n, float* out_planes3,
+                             hipStream_t s) {
+    if (n <= 0) return;
+    const int blocks = std::min((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_export_positions, dim3(blocks), dim3(256), 0, s, part, stride, orig, n, out_planes3);
 }
 
 }  // namespace mpm

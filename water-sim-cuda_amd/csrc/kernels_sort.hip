@@ -207,27 +207,14 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_apply(const uint32_t* in, u
 }
 
 // ---- keys ----
-__global__ __launch_bounds__(256) void k_keys_slots(const ChunkDesc* __restrict__ ch, int chunk_cap,
-                                                    const float* __restrict__ part, size_t stride,
-                                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    const ChunkDesc d = ch[blockIdx.x];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int S = (d.count + 63) >> 6;
-    for (int j = wave; j < S; j += 4) {
-        const int q = lane * S + j;
-        if (q >= d.count) continue;
-        const size_t slot = (size_t)blockIdx.x * chunk_cap + (size_t)j * 64 + lane;
-        keys[d.start + q] = base_key(part[A_PX * stride + slot], part[A_PY * stride + slot], part[A_PZ * stride + slot]);
-        vals[d.start + q] = (uint32_t)slot;
-    }
-}
-
+// Keys of position planes (vals = index): the engine's base-cell keys
+// (internal) or the reference's setKeys keys.
 __global__ void k_keys_linear(const float* __restrict__ px, const float* __restrict__ py,
-                              const float* __restrict__ pz, int n, int internal, uint32_t* __restrict__ keys,
+                              const float* __restrict__ pz, int n, int internal, DevCfg c, uint32_t* __restrict__ keys,
                               uint32_t* __restrict__ vals) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    keys[i] = internal ? base_key(px[i], py[i], pz[i]) : sort_key(px[i], py[i], pz[i]);
+    keys[i] = internal ? base_key(c, px[i], py[i], pz[i]) : sort_key(px[i], py[i], pz[i]);
     vals[i] = (uint32_t)i;
 }
 
@@ -268,41 +255,71 @@ __global__ void k_chunk_desc(const int* __restrict__ seg_start, const uint32_t* 
     const uint32_t k = keys[st];  // base_key: Morton code of the base cell
     const int cx = (int)(compact_bits(k) & ~3u), cy = (int)(compact_bits(k >> 1) & ~3u),
               cz = (int)(compact_bits(k >> 2) & ~3u);
-    const int o = TILE_MARGIN;
     int c = (int)first[s];
     for (int off = 0; off < len; off += cap, ++c) {
         if (c >= limit) return;
         ChunkDesc d;
         d.start = st + off;
         d.count = min(cap, len - off);
-        d.ox = cx - o;
-        d.oy = cy - o;
-        d.oz = cz - o;
-        d.pad0 = d.pad1 = d.pad2 = 0;
+        d.ox = cx - TILE_MARGIN;
+        d.oy = cy - TILE_MARGIN;
+        d.oz = cz - TILE_MARGIN;
+        d.S = 0;  // set by the reorder
+        d.pad0 = d.pad1 = 0;
         chunks[c] = d;
     }
 }
 
-__global__ __launch_bounds__(256) void k_reorder(const ChunkDesc* __restrict__ ch, int chunk_cap,
-                                                 const uint32_t* __restrict__ vals, const float* __restrict__ src,
-                                                 size_t src_stride, const int* __restrict__ src_orig,
-                                                 float* __restrict__ dst, size_t dst_stride,
-                                                 int* __restrict__ dst_orig, uint32_t mask) {
-    const ChunkDesc d = ch[blockIdx.x];
+// One wave per chunk: finds the chunk's groups of equal base-cell keys (at
+// most 64: one 4^3 block), then gathers the particles from their old slots
+// (vals) into rounds -- round r takes the r-th member of every group larger
+// than r -- and records the round sizes and the round count.
+__global__ __launch_bounds__(256) void k_reorder(ChunkDesc* __restrict__ ch, const uint32_t* __restrict__ nchunks,
+                                                 const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                 const float* __restrict__ src, size_t src_stride,
+                                                 const int* __restrict__ src_orig, float* __restrict__ dst,
+                                                 size_t dst_stride, int* __restrict__ dst_orig,
+                                                 uint8_t* __restrict__ rounds, uint32_t mask) {
+    __shared__ int gstart_lds[4][65];
+    const uint32_t nch = *nchunks;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int S = (d.count + 63) >> 6;
-    for (int j = wave; j < S; j += 4) {
-        const int q = lane * S + j;
-        const size_t slot = (size_t)blockIdx.x * chunk_cap + (size_t)j * 64 + lane;
-        if (q >= d.count) {
-            dst_orig[slot] = -1;
-            continue;
+    int* gs = gstart_lds[wave];
+    for (uint32_t ci = blockIdx.x * 4u + wave; ci < nch; ci += gridDim.x * 4u) {
+        const ChunkDesc d = ch[ci];
+        int ng = 0;
+        for (int base = 0; base < d.count; base += 64) {
+            const int q = base + lane;
+            const bool head = q < d.count && (q == 0 || keys[d.start + q] != keys[d.start + q - 1]);
+            const uint64_t m = __ballot(head);
+            if (head) gs[ng + __popcll(m & lanemask_lt(lane))] = q;
+            ng += __popcll(m);
         }
-        const uint32_t si = vals[d.start + q];
+        if (lane == 0) gs[ng] = d.count;
+        // The wave's LDS operations execute in order; the clobber keeps the
+        // compiler from moving the reads above the writes.
+        asm volatile("" ::: "memory");
+        const int g0 = lane < ng ? gs[lane] : 0;
+        const int gsize = lane < ng ? gs[lane + 1] - g0 : 0;
+        int S = gsize;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) S = max(S, __shfl_xor(S, o));
+        int off = 0;
+        for (int r = 0; r < S; ++r) {
+            const bool has = gsize > r;
+            const uint64_t m = __ballot(has);
+            if (has) {
+                const uint32_t si = vals[d.start + g0 + r];
+                const size_t slot = (size_t)d.start + off + __popcll(m & lanemask_lt(lane));
 #pragma unroll 4
-        for (int a = 0; a < A_STATE; ++a)
-            if (mask & (1u << a)) dst[(size_t)a * dst_stride + slot] = src[(size_t)a * src_stride + si];
-        dst_orig[slot] = src_orig ? src_orig[si] : (int)si;
+                for (int a = 0; a < A_STATE; ++a)
+                    if (mask & (1u << a)) dst[(size_t)a * dst_stride + slot] = src[(size_t)a * src_stride + si];
+                dst_orig[slot] = src_orig ? src_orig[si] : (int)si;
+            }
+            if (lane == 0) rounds[d.start + r] = (uint8_t)__popcll(m);
+            off += __popcll(m);
+        }
+        if (lane == 0) ch[ci].S = S;
+        asm volatile("" ::: "memory");  // gs is reused by the wave's next chunk
     }
 }
 
@@ -323,30 +340,29 @@ void scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* p
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)tiles), dim3(SC_THREADS), 0, s, in, out, n, partial);
 }
 
-void launch_keys_slots(const ChunkDesc* ch, int nchunks, int chunk_cap, const float* part, size_t stride,
-                       uint32_t* keys, uint32_t* vals, hipStream_t s) {
-    if (nchunks <= 0) return;
-    hipLaunchKernelGGL(k_keys_slots, dim3(nchunks), dim3(256), 0, s, ch, chunk_cap, part, stride, keys, vals);
-}
-
-void launch_keys_linear(const float* px, const float* py, const float* pz, int n, bool internal, uint32_t* keys,
-                        uint32_t* vals, hipStream_t s) {
+void launch_keys_linear(const float* px, const float* py, const float* pz, int n, bool internal, const DevCfg& c,
+                        uint32_t* keys, uint32_t* vals, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_keys_linear, dim3((n + 255) / 256), dim3(256), 0, s, px, py, pz, n, internal ? 1 : 0, keys,
+    hipLaunchKernelGGL(k_keys_linear, dim3((n + 255) / 256), dim3(256), 0, s, px, py, pz, n, internal ? 1 : 0, c, keys,
                        vals);
 }
 
-int radix_sort_pairs(SortScratch& sc, int n, hipStream_t s) {
+int radix_sort_pairs(SortScratch& sc, int n, int key_bits, hipStream_t s) {
     if (n <= 1) return 0;
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, sc.counters, 0, 0u);
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, sc.counters, 1, 0xFFFFFFFFu);
-    int blocks = (n + 255) / 256;
-    if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(k_key_or_and, dim3(blocks), dim3(256), 0, s, sc.keys[0], n, sc.counters);
-    uint32_t oa[2] = {0u, 0u};
-    hipMemcpyAsync(oa, sc.counters, sizeof(oa), hipMemcpyDeviceToHost, s);
-    hipStreamSynchronize(s);
-    const uint32_t diff = oa[0] ^ oa[1];
+    uint32_t diff;
+    if (key_bits > 0) {
+        diff = key_bits >= 32 ? 0xFFFFFFFFu : ((1u << key_bits) - 1u);
+    } else {
+        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, sc.counters, 0, 0u);
+        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, sc.counters, 1, 0xFFFFFFFFu);
+        int blocks = (n + 255) / 256;
+        if (blocks > 1024) blocks = 1024;
+        hipLaunchKernelGGL(k_key_or_and, dim3(blocks), dim3(256), 0, s, sc.keys[0], n, sc.counters);
+        uint32_t oa[2] = {0u, 0u};
+        hipMemcpyAsync(oa, sc.counters, sizeof(oa), hipMemcpyDeviceToHost, s);
+        hipStreamSynchronize(s);
+        diff = oa[0] ^ oa[1];
+    }
     const int ntiles = (n + RS_TILE - 1) / RS_TILE;
     int r = 0;
     for (int shift = 0; shift < 32; shift += 8) {
@@ -375,12 +391,13 @@ void build_chunks(SortScratch& sc, int r, int n, int chunk_cap, ChunkDesc* chunk
                        sc.keys[r], chunks, chunk_limit);
 }
 
-void launch_reorder(const ChunkDesc* ch, int nchunks, int chunk_cap, const uint32_t* vals_sorted, const float* src,
-                    size_t src_stride, const int* src_orig, float* dst, size_t dst_stride, int* dst_orig,
-                    uint32_t attr_mask, hipStream_t s) {
-    if (nchunks <= 0) return;
-    hipLaunchKernelGGL(k_reorder, dim3(nchunks), dim3(256), 0, s, ch, chunk_cap, vals_sorted, src, src_stride, src_orig,
-                       dst, dst_stride, dst_orig, attr_mask);
+void launch_reorder(ChunkDesc* ch, const uint32_t* nchunks_dev, int chunk_est, const uint32_t* keys_sorted,
+                    const uint32_t* vals_sorted, const float* src, size_t src_stride, const int* src_orig, float* dst,
+                    size_t dst_stride, int* dst_orig, uint8_t* rounds, uint32_t attr_mask, hipStream_t s) {
+    int blocks = (chunk_est + 3) / 4;
+    blocks = blocks < 1 ? 1 : (blocks > 65535 ? 65535 : blocks);
+    hipLaunchKernelGGL(k_reorder, dim3(blocks), dim3(256), 0, s, ch, nchunks_dev, keys_sorted, vals_sorted, src,
+                       src_stride, src_orig, dst, dst_stride, dst_orig, rounds, attr_mask);
 }
 
 }  // namespace mpm

@@ -249,9 +249,15 @@ template <int MODE>
 __global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
     constexpr bool DO_G2P = (MODE & XFER_G2P) != 0;
     constexpr bool DO_P2G = (MODE & XFER_P2G) != 0;
-    const int b = blockIdx.x;
-    const int ci = (b & 7) * a.per_xcd + (b >> 3);  // XCD-contiguous chunk ranges
-    if (ci >= a.nchunks) return;
+    // Workgroups are dispatched to the 8 XCDs round-robin; chunk ranges are
+    // split into 8 contiguous parts so that neighbouring chunks (which share
+    // tile nodes) run on the same XCD and share its L2.
+    const uint32_t nch = *a.nchunks;
+    const int per = (int)((nch + 7u) >> 3);
+    const int xcd = blockIdx.x & 7, nl = gridDim.x >> 3;
+    for (int t = blockIdx.x >> 3; t < per; t += nl) {
+    const int ci = xcd * per + t;
+    if (ci >= (int)nch) break;
     const ChunkDesc d = a.chunks[ci];
     const DevCfg& c = a.c;
 
@@ -280,19 +286,25 @@ __global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
     }
     __syncthreads();
 
-    const int S = (d.count + 63) >> 6;
-    // Round j handles rank lane*S + j.  The next round's particle is loaded
-    // while the current one is processed, hiding HBM latency.
-    const uint32_t slot0 = (uint32_t)ci * (uint32_t)a.chunk_cap + (uint32_t)lane;
+    const int S = d.S;
+    // Round j is the n_j particles in slots [start + off_j, start + off_j + n_j).
+    // The next round's particle is loaded while the current one is
+    // processed, hiding HBM latency.
+    int nxt_n = (int)a.rounds[d.start];
+    int nxt_off = 0;
     Particle nxt;
-    bool nxt_valid = lane * S < d.count;
-    if (nxt_valid) load_particle(a.part, a.stride, slot0, nxt, !DO_G2P);
+    bool nxt_valid = lane < nxt_n;
+    uint32_t nxt_slot = (uint32_t)(d.start + lane);
+    if (nxt_valid) load_particle(a.part, a.stride, nxt_slot, nxt, !DO_G2P);
     for (int j = 0; j < S; ++j) {
         const bool valid = nxt_valid;
-        const uint32_t slot = slot0 + (uint32_t)j * 64u;
+        const uint32_t slot = nxt_slot;
         Particle p = nxt;
-        nxt_valid = j + 1 < S && lane * S + j + 1 < d.count;
-        if (nxt_valid) load_particle(a.part, a.stride, slot + 64u, nxt, !DO_G2P);
+        nxt_off += nxt_n;
+        nxt_n = j + 1 < S ? (int)a.rounds[d.start + j + 1] : 0;
+        nxt_valid = lane < nxt_n;
+        nxt_slot = (uint32_t)(d.start + nxt_off + lane);
+        if (nxt_valid) load_particle(a.part, a.stride, nxt_slot, nxt, !DO_G2P);
         if (valid) {
             if constexpr (DO_G2P) {
                 const int bx = base_of(p.x), by = base_of(p.y), bz = base_of(p.z);
@@ -363,29 +375,34 @@ __global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
                     });
                 }
             }
-            // Lanes with distinct base nodes update the arena together; lanes
-            // sharing a base node take turns.  Each pending lane writes its
-            // id into the claim byte of its base slot and reads it back: one
-            // writer per slot wins (LDS executes the wave's write before its
-            // read), and the winners of a turn have distinct base slots.
+            // Lanes with distinct base nodes update the arena together: each
+            // pending lane writes its id into the claim byte of its base slot
+            // and reads it back; one writer per slot wins (LDS executes the
+            // wave's write before its read) and the others retry in the next
+            // pass.  A round holds distinct base cells when the chunk is
+            // laid out, so extra passes only come from particles whose base
+            // cell changed since the last re-sort.
             bool pending = in_tile;
+            int pass = 0;
             while (__ballot(pending)) {
                 if (pending) claim[li] = (uint8_t)lane;
                 asm volatile("" ::: "memory");
                 const bool mine = pending && claim[li] == (uint8_t)lane;
                 asm volatile("" ::: "memory");
                 if (mine) {
+                    if (pass > 0) atomicAdd(a.overflow + 1, 1u);
                     float4* base = ar + li;
                     p2g_scatter<true>(c, p, pq, [&](int i, int jj, int k, auto contrib) {
                         float4* n = base + (i + TILE_PY * jj + TILE_PZ * k);
                         float4 v = *n;
-                        const float4 d = contrib();
-                        v.x += d.x; v.y += d.y; v.z += d.z; v.w += d.w;
+                        const float4 dd = contrib();
+                        v.x += dd.x; v.y += dd.y; v.z += dd.z; v.w += dd.w;
                         *n = v;
                         asm volatile("" ::: "memory");  // keep the 27 updates in program order
                     });
                 }
                 pending = pending && !mine;
+                ++pass;
             }
         }
     }
@@ -407,14 +424,16 @@ __global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
                 }
             }
         }
+        __syncthreads();  // the next chunk reuses the tile and the arena
+    }
     }
 }
 
 }  // namespace
 
-void launch_transfer(int mode, const TransferArgs& a, hipStream_t s) {
-    if (a.nchunks <= 0) return;
-    dim3 grid(8 * a.per_xcd), block(TRANSFER_THREADS);
+void launch_transfer(int mode, const TransferArgs& a, int grid_chunks, hipStream_t s) {
+    const int per = (grid_chunks + 7) / 8;
+    dim3 grid(8 * (per < 1 ? 1 : per)), block(TRANSFER_THREADS);
     switch (mode) {
         case XFER_P2G: hipLaunchKernelGGL(k_transfer<XFER_P2G>, grid, block, 0, s, a); break;
         case XFER_G2P: hipLaunchKernelGGL(k_transfer<XFER_G2P>, grid, block, 0, s, a); break;
