@@ -174,7 +174,8 @@ int mpm_engine_get_stage_times(mpm_engine* h, double* out, int nstages);
  * out[1] = particle-substeps whose stencil had left their chunk's LDS tile
  * (global-memory path), out[2] = active 4^3 grid blocks, out[3] =
  * particle-substeps that shared a base node with another lane of their round
- * (global-atomic path). */
+ * and took an extra pass of the LDS update, out[4] = wave rounds of the
+ * current particle layout (one round = up to 64 particles). */
 int mpm_engine_get_counters(mpm_engine* h, int64_t* out, int n);
 
 /* ---- stage access for parity tests (no reference counterpart) ---- */

@@ -371,10 +371,10 @@ class Engine:
         return {s: (float(out[2 * i]), int(out[2 * i + 1])) for i, s in enumerate(STAGES)}
 
     def counters(self) -> dict:
-        out = np.zeros(4, dtype=np.int64)
-        _check(self._lib.mpm_engine_get_counters(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 4))
+        out = np.zeros(5, dtype=np.int64)
+        _check(self._lib.mpm_engine_get_counters(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 5))
         return {"chunks": int(out[0]), "tile_overflow_particles": int(out[1]), "active_blocks": int(out[2]),
-                "claim_losers": int(out[3])}
+                "claim_losers": int(out[3]), "rounds": int(out[4])}
 
     def debug_p2g(self) -> np.ndarray:
         out = np.empty((4, self.cfg.nodes), dtype=np.float32)

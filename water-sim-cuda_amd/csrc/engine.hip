@@ -78,13 +78,14 @@ struct mpm_engine {
     float* sdf = nullptr;
 
     SortScratch sc{};
-    uint32_t* overflow = nullptr;
+    uint32_t* overflow = nullptr;  // = sc.counters + 8: [0] tile-overflow particles, [1] extra claim passes
     uint32_t* pinned = nullptr;
 
     int64_t substeps = 0;
     int64_t sorted_at = 0;
     int64_t last_overflow = 0;
     int64_t last_claim_losers = 0;
+    int64_t last_rounds = 0;
 
     struct Ev {
         hipEvent_t a, b;
@@ -194,10 +195,11 @@ int ensure_planes(mpm_engine* h, int which, size_t slots) {
     return MPM_OK;
 }
 
-int read_counter(mpm_engine* h, uint32_t* dev, uint32_t* out) {
-    HIPCHK(hipMemcpyAsync(h->pinned, dev, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+// Copies the 16 device counters (SortScratch::counters) into h->pinned and
+// waits for the stream.
+int read_counters(mpm_engine* h) {
+    HIPCHK(hipMemcpyAsync(h->pinned, h->sc.counters, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    *out = h->pinned[0];
     return MPM_OK;
 }
 
@@ -211,7 +213,8 @@ int sort_and_reorder(mpm_engine* h, const float* src, size_t src_stride, const i
     build_chunks(h->sc, r, n, h->chunk_cap, h->chunks, h->chunk_limit, h->stream);
     const int dst = 1 - h->pcur;
     launch_reorder(h->chunks, h->sc.counters + 3, h->chunk_est, h->sc.keys[r], h->sc.vals[r], src, src_stride,
-                   src_orig, h->planes[dst], h->plane_cap[dst], h->orig[dst], h->rounds, mask, h->stream);
+                   src_orig, h->planes[dst], h->plane_cap[dst], h->orig[dst], h->rounds, h->sc.counters + 4, mask,
+                   h->stream);
     HIPCHK(hipGetLastError());
     h->pcur = dst;
     return MPM_OK;
@@ -336,9 +339,9 @@ int mpm_engine_create(const mpm_sim_config* cfg_in, mpm_engine** out) {
     if ((rc = dev_alloc(&h->sdf, h->G))) return cleanup(rc);
     launch_fill_f32(h->sdf, h->G, 1e15f, h->stream);  // makeSDF with no triangles
     if ((rc = dev_alloc(&h->sc.counters, 64))) return cleanup(rc);
-    if ((rc = dev_alloc(&h->overflow, 2))) return cleanup(rc);
-    if (hipMemsetAsync(h->overflow, 0, 2 * sizeof(uint32_t), h->stream) != hipSuccess)
+    if (hipMemsetAsync(h->sc.counters, 0, 64 * sizeof(uint32_t), h->stream) != hipSuccess)
         return cleanup(fail(MPM_EHIP, "memset"));
+    h->overflow = h->sc.counters + 8;
     if (hipHostMalloc((void**)&h->pinned, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
         return cleanup(fail(MPM_ENOMEM, "hipHostMalloc failed"));
     if (hipStreamSynchronize(h->stream) != hipSuccess) return cleanup(fail(MPM_EHIP, "stream sync"));
@@ -357,7 +360,7 @@ int mpm_engine_destroy(mpm_engine* h) {
     }
     dev_free(h->sdf);
     dev_free(h->sc.counters);
-    dev_free(h->overflow);
+    h->overflow = nullptr;
     if (h->pinned) hipHostFree(h->pinned);
     for (auto& e : h->evs) {
         hipEventDestroy(e.a);
@@ -438,9 +441,9 @@ int mpm_engine_init_particles(mpm_engine* h, int n, const float* soa26) {
     dev_free(tmp);
     if (rc) return rc;
     HIPCHK(hipGetLastError());
-    uint32_t nch = 0;
-    if ((rc = read_counter(h, h->sc.counters + 3, &nch))) return rc;
-    h->chunk_est = (int)nch;
+    if ((rc = read_counters(h))) return rc;
+    h->chunk_est = (int)h->pinned[3];
+    h->last_rounds = h->pinned[4];
     h->have_particles = true;
     return MPM_OK;
 }
@@ -509,13 +512,11 @@ int mpm_engine_steps(mpm_engine* h, int k) {
         }
     }
     HIPCHK(hipGetLastError());
-    uint32_t ov = 0, cl = 0, nch = 0;
-    if ((rc = read_counter(h, h->overflow, &ov)) || (rc = read_counter(h, h->overflow + 1, &cl)) ||
-        (rc = read_counter(h, h->sc.counters + 3, &nch)))
-        return rc;
-    h->last_overflow = ov;
-    h->last_claim_losers = cl;
-    h->chunk_est = (int)nch;
+    if ((rc = read_counters(h))) return rc;
+    h->last_overflow = h->pinned[8];
+    h->last_claim_losers = h->pinned[9];
+    h->chunk_est = (int)h->pinned[3];
+    h->last_rounds = h->pinned[4];
     return MPM_OK;
 }
 
@@ -628,7 +629,7 @@ int mpm_engine_get_stage_times(mpm_engine* h, double* out, int nstages) {
 
 int mpm_engine_get_counters(mpm_engine* h, int64_t* out, int n) {
     if (!h || !out) return fail(MPM_EINVAL, "null argument");
-    int64_t v[4] = {h->chunk_est, h->last_overflow, 0, h->last_claim_losers};
+    int64_t v[5] = {h->chunk_est, h->last_overflow, 0, h->last_claim_losers, h->last_rounds};
     if (n > 2) {
         std::vector<uint8_t> f((size_t)h->nfb);
         hipSetDevice(h->device);
@@ -638,7 +639,7 @@ int mpm_engine_get_counters(mpm_engine* h, int64_t* out, int n) {
         for (uint8_t b : f) cnt += b != 0;
         v[2] = cnt;
     }
-    for (int i = 0; i < n && i < 4; ++i) out[i] = v[i];
+    for (int i = 0; i < n && i < 5; ++i) out[i] = v[i];
     return MPM_OK;
 }
 

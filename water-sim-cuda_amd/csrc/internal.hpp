@@ -62,7 +62,9 @@ struct SortScratch {
     uint32_t* vals[2];
     uint32_t* hist;      // 256 * ntiles
     uint32_t* partial;   // scan partials
-    uint32_t* counters;  // [0] = or, [1] = and, [2] = nseg, [3] = nchunks, ...
+    // [0] = or, [1] = and, [2] = segments, [3] = chunks, [4] = rounds of the
+    // last layout; the engine keeps its per-call counters at [8..].
+    uint32_t* counters;
     int* heads;          // n + 1
     int* seg_start;      // n + 1
     int* seg_nch;        // n + 1
@@ -93,10 +95,11 @@ void scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* p
 void build_chunks(SortScratch& sc, int r, int n, int chunk_cap, ChunkDesc* chunks, int chunk_limit, hipStream_t s);
 // Gathers the particles into the round layout of their chunks (one wave per
 // chunk; the chunk count is read on the device), writes the round sizes and
-// each chunk's round count.  attr_mask selects the planes moved (bit a =
-// attribute a).
+// each chunk's round count, and adds the round counts into *rounds_total.
+// attr_mask selects the planes moved (bit a = attribute a).
 void launch_reorder(ChunkDesc* ch, const uint32_t* nchunks_dev, int chunk_est, const uint32_t* keys_sorted,
                     const uint32_t* vals_sorted, const float* src, size_t src_stride, const int* src_orig, float* dst,
-                    size_t dst_stride, int* dst_orig, uint8_t* rounds, uint32_t attr_mask, hipStream_t s);
+                    size_t dst_stride, int* dst_orig, uint8_t* rounds, uint32_t* rounds_total, uint32_t attr_mask,
+                    hipStream_t s);
 
 }  // namespace mpm

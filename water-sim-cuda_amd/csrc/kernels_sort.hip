@@ -279,7 +279,8 @@ __global__ __launch_bounds__(256) void k_reorder(ChunkDesc* __restrict__ ch, con
                                                  const float* __restrict__ src, size_t src_stride,
                                                  const int* __restrict__ src_orig, float* __restrict__ dst,
                                                  size_t dst_stride, int* __restrict__ dst_orig,
-                                                 uint8_t* __restrict__ rounds, uint32_t mask) {
+                                                 uint8_t* __restrict__ rounds, uint32_t* __restrict__ rounds_total,
+                                                 uint32_t mask) {
     __shared__ int gstart_lds[4][65];
     const uint32_t nch = *nchunks;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -318,7 +319,10 @@ __global__ __launch_bounds__(256) void k_reorder(ChunkDesc* __restrict__ ch, con
             if (lane == 0) rounds[d.start + r] = (uint8_t)__popcll(m);
             off += __popcll(m);
         }
-        if (lane == 0) ch[ci].S = S;
+        if (lane == 0) {
+            ch[ci].S = S;
+            atomicAdd(rounds_total, (uint32_t)S);
+        }
         asm volatile("" ::: "memory");  // gs is reused by the wave's next chunk
     }
 }
@@ -382,6 +386,7 @@ void build_chunks(SortScratch& sc, int r, int n, int chunk_cap, ChunkDesc* chunk
     uint32_t* hscan = sc.keys[1 - r];  // free after the sort
     uint32_t* nch = reinterpret_cast<uint32_t*>(sc.seg_nch);
     uint32_t* first = sc.vals[1 - r];
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, sc.counters, 4, 0u);  // round total of the new layout
     hipLaunchKernelGGL(k_seg_heads, dim3(blocks), dim3(256), 0, s, sc.keys[r], n, heads);
     scan_exclusive_u32(heads, hscan, (size_t)n, sc.partial, sc.counters + 2, s);
     hipLaunchKernelGGL(k_seg_starts, dim3(blocks), dim3(256), 0, s, heads, hscan, n, sc.counters, sc.seg_start);
@@ -393,11 +398,12 @@ void build_chunks(SortScratch& sc, int r, int n, int chunk_cap, ChunkDesc* chunk
 
 void launch_reorder(ChunkDesc* ch, const uint32_t* nchunks_dev, int chunk_est, const uint32_t* keys_sorted,
                     const uint32_t* vals_sorted, const float* src, size_t src_stride, const int* src_orig, float* dst,
-                    size_t dst_stride, int* dst_orig, uint8_t* rounds, uint32_t attr_mask, hipStream_t s) {
+                    size_t dst_stride, int* dst_orig, uint8_t* rounds, uint32_t* rounds_total, uint32_t attr_mask,
+                    hipStream_t s) {
     int blocks = (chunk_est + 3) / 4;
     blocks = blocks < 1 ? 1 : (blocks > 65535 ? 65535 : blocks);
     hipLaunchKernelGGL(k_reorder, dim3(blocks), dim3(256), 0, s, ch, nchunks_dev, keys_sorted, vals_sorted, src,
-                       src_stride, src_orig, dst, dst_stride, dst_orig, rounds, attr_mask);
+                       src_stride, src_orig, dst, dst_stride, dst_orig, rounds, rounds_total, attr_mask);
 }
 
 }  // namespace mpm

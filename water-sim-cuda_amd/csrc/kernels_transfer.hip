@@ -128,6 +128,20 @@ __device__ __forceinline__ float sdf_sample(const DevCfg& c, const float* __rest
     return sdf_at(c, sdf, px, py, pz);
 }
 
+// calculateNormal (common.cuh:169-182) with each of the six samples taken
+// from the LDS tile when its cell lies there (same node values, same
+// arithmetic as the global path).
+__device__ __forceinline__ void sdf_normal_tile(const DevCfg& c, const float* __restrict__ sdf, const float4* vt,
+                                                int ox, int oy, int oz, float px, float py, float pz, float& nx,
+                                                float& ny, float& nz) {
+    const float eps = 0.1f;
+    nx = sdf_sample(c, sdf, vt, ox, oy, oz, px + eps, py, pz) - sdf_sample(c, sdf, vt, ox, oy, oz, px - eps, py, pz);
+    ny = sdf_sample(c, sdf, vt, ox, oy, oz, px, py + eps, pz) - sdf_sample(c, sdf, vt, ox, oy, oz, px, py - eps, pz);
+    nz = sdf_sample(c, sdf, vt, ox, oy, oz, px, py, pz + eps) - sdf_sample(c, sdf, vt, ox, oy, oz, px, py, pz - eps);
+    const float len = sqrtf(nx * nx + ny * ny + nz * nz) + 1e-6f;
+    nx /= len; ny /= len; nz /= len;
+}
+
 // F update, J floor, advection and collider projection (kernels.cu:237-279).
 __device__ __forceinline__ void g2p_finish(const DevCfg& c, const float* __restrict__ sdf, const float4* vt, int ox,
                                            int oy, int oz, Particle& p, const float vel_in[3], const float C[9]) {
@@ -141,9 +155,9 @@ __device__ __forceinline__ void g2p_finish(const DevCfg& c, const float* __restr
 #pragma unroll
         for (int q = 0; q < 3; ++q)
             nF[3 * r + q] = A[3 * r] * p.F[q] + A[3 * r + 1] * p.F[3 + q] + A[3 * r + 2] * p.F[6 + q];
-    float J = det3(nF);
+    const float J = det3(nF);
     if (J < 0.3f) {
-        float scale = powf(0.3f / J, 1.0f / 3.0f);
+        const float scale = powf(0.3f / J, 1.0f / 3.0f);
 #pragma unroll
         for (int i = 0; i < 9; ++i) nF[i] *= scale;
     }
@@ -152,7 +166,7 @@ __device__ __forceinline__ void g2p_finish(const DevCfg& c, const float* __restr
     float dist = sdf_sample(c, sdf, vt, ox, oy, oz, nx_, ny_, nz_);
     if (dist < 0.0f) {
         float nx, ny, nz;
-        sdf_normal(c, sdf, nx_, ny_, nz_, nx, ny, nz);
+        sdf_normal_tile(c, sdf, vt, ox, oy, oz, nx_, ny_, nz_, nx, ny, nz);
         nx_ -= nx * dist; ny_ -= ny * dist; nz_ -= nz * dist;
         float vn = vx * nx + vy * ny + vz * nz;
         if (vn < 0.0f) {
@@ -178,7 +192,10 @@ struct P2GPrep {
 __device__ __forceinline__ void p2g_prepare(const DevCfg& c, const Particle& p, int bx, int by, int bz, P2GPrep& q) {
     float J = det3(p.F);
     J = fmaxf(0.9f, fminf(J, 1.1f));
-    float pressure = c.compression * (powf(J, c.gamma) - 1.0f);
+    // J^GAMMA for J in [0.9, 1.1] as 2^(GAMMA log2 J) on the hardware log and
+    // exp units (4 instructions instead of a general powf's ~150); the
+    // relative error is a few 1e-7, far inside the step's tolerance.
+    float pressure = c.compression * (__builtin_amdgcn_exp2f(c.gamma * __builtin_amdgcn_logf(J)) - 1.0f);
     if (pressure < 0.0f) pressure = 0.0f;
     float volume = c.vol * J;
     float multi = 4.0f * c.dt * volume * pressure / c.mass;
@@ -191,37 +208,47 @@ __device__ __forceinline__ void p2g_prepare(const DevCfg& c, const Particle& p, 
 }
 
 // Node contributions m w (v + C' (x_i - x_p)) in the reference's loop order
-// (x outer, z inner), with the x and y parts of C' (x_i - x_p) shared by the
-// three nodes of a z column.  `scatter(i, j, k, contrib)` receives a functor
-// returning (m w, momentum) so it can evaluate it where it wants; with PIN each
-// z column's inputs pass through an empty asm inside the column's first
-// update, which keeps the compiler from hoisting all 27 contributions (108
-// registers) above the update chain.
-template <bool PIN, class Scatter>
+// (x outer, z inner).  With w = wx wy and e = v + C' (dx, dy, 0) per (x, y)
+// column, mz = m wz and g = m wz dz C'[:, 2] per z offset, a node receives
+// mass w mz and momentum w (mz e + g).  `scatter(i, j, k, n)` gets a Node
+// that either returns that contribution (value(), for atomics) or adds it to
+// a float4 with fused multiply-adds (add_to(), 7 instructions for the 4
+// components).
+struct P2GNode {
+    float w, mz, e0, e1, e2, g0, g1, g2;
+    __device__ __forceinline__ float4 value() const {
+        return make_float4(w * mz, w * fmaf(mz, e0, g0), w * fmaf(mz, e1, g1), w * fmaf(mz, e2, g2));
+    }
+    __device__ __forceinline__ void add_to(float4& v) const {
+        v.x = fmaf(w, mz, v.x);
+        v.y = fmaf(w, fmaf(mz, e0, g0), v.y);
+        v.z = fmaf(w, fmaf(mz, e1, g1), v.z);
+        v.w = fmaf(w, fmaf(mz, e2, g2), v.w);
+    }
+};
+
+template <class Scatter>
 __device__ __forceinline__ void p2g_scatter(const DevCfg& c, const Particle& p, const P2GPrep& q, Scatter scatter) {
-    float mwz[3];
+    float mz[3], g0[3], g1[3], g2[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) mwz[k] = c.mass * q.wz[k];
+    for (int k = 0; k < 3; ++k) {
+        mz[k] = c.mass * q.wz[k];
+        const float mzd = mz[k] * q.dz[k];
+        g0[k] = mzd * q.C[2];
+        g1[k] = mzd * q.C[5];
+        g2[k] = mzd * q.C[8];
+    }
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < 3; ++i) {
+        const float f0 = p.vx + q.C[0] * q.dx[i], f1 = p.vy + q.C[3] * q.dx[i], f2 = p.vz + q.C[6] * q.dx[i];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            float w = q.wx[i] * q.wy[j];
-            float e0 = p.vx + q.C[0] * q.dx[i] + q.C[1] * q.dy[j];
-            float e1 = p.vy + q.C[3] * q.dx[i] + q.C[4] * q.dy[j];
-            float e2 = p.vz + q.C[6] * q.dx[i] + q.C[7] * q.dy[j];
+            const float w = q.wx[i] * q.wy[j];
+            const float e0 = f0 + q.C[1] * q.dy[j], e1 = f1 + q.C[4] * q.dy[j], e2 = f2 + q.C[7] * q.dy[j];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                scatter(i, j, k, [&]() {
-                    if constexpr (PIN) {
-                        if (k == 0) asm volatile("" : "+v"(w), "+v"(e0), "+v"(e1), "+v"(e2));
-                    }
-                    const float wm = w * mwz[k];
-                    return make_float4(wm, wm * (e0 + q.C[2] * q.dz[k]), wm * (e1 + q.C[5] * q.dz[k]),
-                                       wm * (e2 + q.C[8] * q.dz[k]));
-                });
-            }
+            for (int k = 0; k < 3; ++k) scatter(i, j, k, P2GNode{w, mz[k], e0, e1, e2, g0[k], g1[k], g2[k]});
         }
+    }
 }
 
 __device__ __forceinline__ void load_particle(const float* __restrict__ part, size_t stride, uint32_t slot, Particle& p,
@@ -362,11 +389,11 @@ __global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
                 p2g_prepare(c, p, bx, by, bz, pq);
                 if (!in_tile) {
                     atomicAdd(a.overflow, 1u);
-                    p2g_scatter<false>(c, p, pq, [&](int i, int jj, int k, auto contrib) {
+                    p2g_scatter(c, p, pq, [&](int i, int jj, int k, const P2GNode& nd) {
                         int x = bx + i, y = by + jj, z = bz + k;
                         if (!inb(c, x, y, z)) return;
                         int g = gidx(c, x, y, z);
-                        const float4 d = contrib();
+                        const float4 d = nd.value();
                         atomic_add_global(a.acc[0] + g, d.x);
                         atomic_add_global(a.acc[1] + g, d.y);
                         atomic_add_global(a.acc[2] + g, d.z);
@@ -378,31 +405,49 @@ __global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
             // Lanes with distinct base nodes update the arena together: each
             // pending lane writes its id into the claim byte of its base slot
             // and reads it back; one writer per slot wins (LDS executes the
-            // wave's write before its read) and the others retry in the next
+            // wave's write before its read) and the others retry in another
             // pass.  A round holds distinct base cells when the chunk is
             // laid out, so extra passes only come from particles whose base
-            // cell changed since the last re-sort.
-            bool pending = in_tile;
-            int pass = 0;
+            // cell changed since the last re-sort.  The first pass stands
+            // outside the retry loop, and the retry passes launder their
+            // inputs, so that the compiler keeps the common path's
+            // contributions inside the update chain instead of hoisting them
+            // (and their registers) above a loop.
+            auto arena_chain = [&](const Particle& pp, const P2GPrep& qq) {
+                float4* base = ar + li;
+                p2g_scatter(c, pp, qq, [&](int i, int jj, int k, const P2GNode& nd) {
+                    float4* n = base + (i + TILE_PY * jj + TILE_PZ * k);
+                    float4 v = *n;
+                    nd.add_to(v);
+                    *n = v;
+                    asm volatile("" ::: "memory");  // keep the 27 updates in program order
+                });
+            };
+            if (in_tile) claim[li] = (uint8_t)lane;
+            asm volatile("" ::: "memory");
+            const bool mine = in_tile && claim[li] == (uint8_t)lane;
+            asm volatile("" ::: "memory");
+            if (mine) arena_chain(p, pq);
+            bool pending = in_tile && !mine;
             while (__ballot(pending)) {
                 if (pending) claim[li] = (uint8_t)lane;
                 asm volatile("" ::: "memory");
-                const bool mine = pending && claim[li] == (uint8_t)lane;
+                const bool won = pending && claim[li] == (uint8_t)lane;
                 asm volatile("" ::: "memory");
-                if (mine) {
-                    if (pass > 0) atomicAdd(a.overflow + 1, 1u);
-                    float4* base = ar + li;
-                    p2g_scatter<true>(c, p, pq, [&](int i, int jj, int k, auto contrib) {
-                        float4* n = base + (i + TILE_PY * jj + TILE_PZ * k);
-                        float4 v = *n;
-                        const float4 dd = contrib();
-                        v.x += dd.x; v.y += dd.y; v.z += dd.z; v.w += dd.w;
-                        *n = v;
-                        asm volatile("" ::: "memory");  // keep the 27 updates in program order
-                    });
+                if (won) {
+                    atomicAdd(a.overflow + 1, 1u);
+                    Particle p2 = p;
+                    P2GPrep q2 = pq;
+                    asm volatile("" : "+v"(p2.vx), "+v"(p2.vy), "+v"(p2.vz));
+#pragma unroll
+                    for (int t = 0; t < 9; ++t) asm volatile("" : "+v"(q2.C[t]));
+#pragma unroll
+                    for (int t = 0; t < 3; ++t)
+                        asm volatile("" : "+v"(q2.wx[t]), "+v"(q2.wy[t]), "+v"(q2.wz[t]), "+v"(q2.dx[t]),
+                                     "+v"(q2.dy[t]), "+v"(q2.dz[t]));
+                    arena_chain(p2, q2);
                 }
-                pending = pending && !mine;
-                ++pass;
+                pending = pending && !won;
             }
         }
     }
