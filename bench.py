@@ -3,8 +3,11 @@
 Prints one JSON line.  A "step" is one MLS-MPM substep (P2G, grid update,
 G2P, and the Morton re-sort every RESORT_EVERY substeps) over all particles,
 with the state resident in HBM.  `value` is particle-substeps per second over
-the K timed substeps (one engine call, so the fused G2P+P2G kernel runs
-K-1 times).  `roofline` describes the dominant kernel (fused G2P+P2G),
+the K timed substeps, run as the reference's frame loop does: SUBSTEPS
+substeps per engine call (40 for the 256^3 channel), so within a frame the
+fused G2P+P2G kernel runs SUBSTEPS-1 times.  `--scene batch64` measures the
+batched 64-scene workload instead; `--graph` replays each frame as a HIP
+graph; `--autotune` tunes CHUNK and LAUNCH_PCT first.  `roofline` describes the dominant kernel (fused G2P+P2G),
 timed with HIP events on the engine's stream inside the timed region.
 `cpu_baseline` times the oracle's OpenMP step on the host cores on a
 bounded sample (a few substeps of the same scene).
@@ -66,6 +69,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events")
+    ap.add_argument("--graph", action="store_true", help="replay each frame's steps() call as a HIP graph")
+    ap.add_argument("--frame", type=int, default=0, help="substeps per steps() call [the scene's SUBSTEPS]")
+    ap.add_argument("--autotune", action="store_true", help="pick CHUNK and LAUNCH_PCT by timing trials first")
+    ap.add_argument("--tune-cache", default=None, help="JSON cache file for --autotune")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "g2p2g_channel256_traffic.txt"),
                     help="rocprofv3-derived HBM bytes per launch of the fused kernel (key: value text)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -86,32 +93,60 @@ def main() -> int:
 
     mpm = importlib.import_module("water-sim-cuda_amd")
     t0 = time.time()
-    sc = mpm.scenes.SCENES[args.scene]()
-    cfg = sc["cfg"]
-    n = sc["particles"].shape[1]
+    batch = args.scene.startswith("batch")
+    if batch:
+        sc = mpm.scenes.batch_scenes(count=int(args.scene[5:] or 64))
+        cfg = sc["cfgs"][0]
+        n = int(sc["counts"].sum())
+        grid_desc = sorted({c.SIZE_X for c in sc["cfgs"]})
+    else:
+        sc = mpm.scenes.SCENES[args.scene]()
+        cfg = sc["cfg"]
+        n = sc["particles"].shape[1]
+        grid_desc = [cfg.SIZE_X, cfg.SIZE_Y, cfg.SIZE_Z]
     resort = args.resort_every or sc["resort_every"]
-    log(f"scene {sc['name']}: n={n} grid={cfg.SIZE_X}^3 (host gen {time.time() - t0:.1f}s)")
+    log(f"scene {sc['name']}: n={n} grid={grid_desc} (host gen {time.time() - t0:.1f}s)")
 
-    opts = {"RESORT_EVERY": resort, "TIMING": 0 if args.no_timing else 1}
+    opts = {"RESORT_EVERY": resort, "TIMING": 0 if args.no_timing else 1, "GRAPH": 1 if args.graph else 0}
     if args.chunk:
         opts["CHUNK"] = args.chunk
-    eng = mpm.Engine(cfg, **opts)
-    eng.init(n, sc["particles"])
     t0 = time.time()
-    if sc.get("sdf") is not None:
-        eng.set_sdf(sc["sdf"])
+    if batch:
+        eng = mpm.Batch(sc["cfgs"], **opts)
+        eng.init(sc["counts"], sc["particles"])
+        for s_, sdf in enumerate(sc["sdfs"]):
+            eng.set_scene_sdf(s_, sdf)
     else:
-        eng.initSDF(sc["triangles"])
-    log(f"sdf ready ({time.time() - t0:.2f}s, {0 if sc.get('triangles') is None else len(sc['triangles'])} tris)")
+        eng = mpm.Engine(cfg, **opts)
+        eng.init(n, sc["particles"])
+        if sc.get("sdf") is not None:
+            eng.set_sdf(sc["sdf"])
+        else:
+            eng.initSDF(sc["triangles"])
+    log(f"engine ready ({time.time() - t0:.2f}s)")
+    tuned = None
+    if args.autotune:
+        from importlib import import_module
 
-    eng.steps(args.warmup)
+        tuned = import_module("water-sim-cuda_amd.autotune").autotune(eng, cache=args.tune_cache)
+        log(f"autotune: CHUNK={tuned['CHUNK']} LAUNCH_PCT={tuned['LAUNCH_PCT']} cached={tuned['cached']} "
+            f"trials {tuned.get('trials')}")
+
+    # The reference's frame loop: SUBSTEPS substeps per call (Simulation.cpp:42-54).
+    frame = args.frame or int(cfg.SUBSTEPS) or 20
+    nframes = max(1, -(-args.steps // frame))
+    wframes = max(1, -(-args.warmup // frame))
+    for _ in range(wframes):  # with --graph the first call captures the frame graph
+        eng.steps(frame)
     eng.synchronize()
     eng.stage_times()  # drop warm-up events
-    log(f"warmup {args.warmup} substeps done")
+    log(f"warmup {wframes} x {frame} substeps done")
+    args.steps, args.warmup = nframes * frame, wframes * frame
 
     eng.synchronize()
     t0 = time.perf_counter()
-    eng.steps(args.steps)
+    for _ in range(nframes):
+        eng.steps(frame)
     eng.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
@@ -141,14 +176,33 @@ def main() -> int:
         from oracle import oracle
 
         nthreads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
-        sdf = eng.get_sdf()
-        st0 = oracle.init_state(cfg, sc["particles"], np.float32)
-        t_one, _ = oracle.time_steps(cfg, st0, sdf, 1, nthreads)
-        k = max(1, min(20, int(args.cpu_seconds / max(t_one, 1e-3))))
-        t_k, _ = oracle.time_steps(cfg, st0, sdf, k, nthreads)
-        cpu = {"value": round(n * k / t_k, 1), "unit": "particle-substeps/s", "cores": nthreads, "kind": "port",
-               "sample": f"{k} substeps of the full {sc['name']} scene (n={n}) with the OpenMP oracle, "
-                         f"{t_k:.1f}s wall"}
+        if batch:
+            # scene by scene (the oracle steps one scene at a time) until the
+            # time budget is used; k = 2 substeps per scene
+            done, t_sum, used = 0, 0.0, 0
+            lo = 0
+            for s_, c_ in enumerate(sc["cfgs"]):
+                hi = lo + int(sc["counts"][s_])
+                st0 = oracle.init_state(c_, np.ascontiguousarray(sc["particles"][:, lo:hi]), np.float32)
+                t_k, _ = oracle.time_steps(c_, st0, sc["sdfs"][s_], 2, nthreads)
+                done += 2 * (hi - lo)
+                t_sum += t_k
+                used += 1
+                lo = hi
+                if t_sum > args.cpu_seconds:
+                    break
+            cpu = {"value": round(done / t_sum, 1), "unit": "particle-substeps/s", "cores": nthreads, "kind": "port",
+                   "sample": f"2 substeps of each of the first {used} of {len(sc['cfgs'])} scenes "
+                             f"({done // 2} particles) with the OpenMP oracle, {t_sum:.1f}s wall"}
+        else:
+            sdf = eng.get_sdf()
+            st0 = oracle.init_state(cfg, sc["particles"], np.float32)
+            t_one, _ = oracle.time_steps(cfg, st0, sdf, 1, nthreads)
+            k = max(1, min(20, int(args.cpu_seconds / max(t_one, 1e-3))))
+            t_k, _ = oracle.time_steps(cfg, st0, sdf, k, nthreads)
+            cpu = {"value": round(n * k / t_k, 1), "unit": "particle-substeps/s", "cores": nthreads, "kind": "port",
+                   "sample": f"{k} substeps of the full {sc['name']} scene (n={n}) with the OpenMP oracle, "
+                             f"{t_k:.1f}s wall"}
         log(f"cpu baseline: {cpu}")
     eng.close()
 
@@ -165,9 +219,11 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": sc["name"], "grid": [cfg.SIZE_X, cfg.SIZE_Y, cfg.SIZE_Z], "particles": n,
-                   "dt": cfg.DT, "resort_every": resort, "description": sc["description"],
-                   "state_finite": finite},
+        "config": {"workload": sc["name"], "grid": grid_desc, "particles": n,
+                   "scenes": len(sc["cfgs"]) if batch else 1, "dt": cfg.DT, "resort_every": resort,
+                   "graph": bool(args.graph), "autotune": None if tuned is None else
+                   {k: tuned[k] for k in ("CHUNK", "LAUNCH_PCT", "cached")},
+                   "description": sc["description"], "state_finite": finite},
         "roofline": roof,
         "cpu_baseline": cpu,
         "stage_ms_per_launch": {k: (round(v[0] / v[1], 5) if v[1] else None) for k, v in st.items()},

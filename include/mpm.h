@@ -113,10 +113,36 @@ int mpm_engine_destroy(mpm_engine* h);
  *   "RESORT_EVERY"  Morton re-sort period in substeps [1 = every substep, as
  *                   engine.cu:57].  The sort changes only the order of the
  *                   float atomics, not the exact-arithmetic result.
- *   "CHUNK"         maximum particles per transfer workgroup [512]
- *   "TIMING"        1 = record per-stage HIP events [0]            */
+ *   "CHUNK"         maximum particles per transfer work item, a multiple of
+ *                   64 in [64, 4096] [512]; changing it re-lays the particles
+ *   "LAUNCH_PCT"    transfer workgroups per 100 chunks, 1..100 [100]; below
+ *                   100 each workgroup loops over several chunks
+ *   "TIMING"        1 = record per-stage HIP events [0]
+ *   "GRAPH"         1 = capture a steps(k) call as a HIP graph once and replay
+ *                   it while the schedule state matches [0]
+ * Read-only (get_option): "GRAPH_REPLAYS" graph launches so far, "SCENES"
+ * scene count. */
 int mpm_engine_set_option(mpm_engine* h, const char* key, int64_t value);
 int mpm_engine_get_option(mpm_engine* h, const char* key, int64_t* value);
+
+/* ---- batched scenes (no reference counterpart) ----
+ * One engine can hold nscenes independent scenes, each with its own config
+ * (grid size, DT, GRAVITY, ...), grid, SDF and particles.  Every stage runs as
+ * one launch over all scenes.  Particles of all scenes form one caller-order
+ * array, scene after scene (get_particles/get_positions return it in that
+ * order); grids and SDFs are concatenated scene after scene
+ * (mpm_batch_scene_info gives the offsets).  mpm_engine_set_sdf/get_sdf and
+ * mpm_engine_get_grid address the concatenation.  Limits: nscenes <= 4096 and
+ * 3*ceil(log2(largest edge)) + ceil(log2(nscenes)) <= 32. */
+int mpm_batch_create(const mpm_sim_config* cfgs, int nscenes, mpm_engine** out);
+/* counts[s] particles per scene from a 26 x sum(counts) SoA buffer. */
+int mpm_batch_init_particles(mpm_engine* h, const int* counts, const float* soa26);
+int mpm_batch_init_sdf_from_triangles(mpm_engine* h, int scene, const mpm_triangle* tris, int nt);
+int mpm_batch_set_sdf(mpm_engine* h, int scene, const float* sdf);
+/* Planes mass, v_x, v_y, v_z of one scene after its last grid update. */
+int mpm_batch_get_grid(mpm_engine* h, int scene, float* grid4_out);
+/* out4 = {first particle, particle count, first grid node, grid nodes}. */
+int mpm_batch_scene_info(mpm_engine* h, int scene, int64_t* out4);
 
 /* ---- particles and collider ---- */
 
@@ -143,6 +169,13 @@ int mpm_engine_step(mpm_engine* h);
 int mpm_engine_steps(mpm_engine* h, int k);
 /* Blocks until the engine's stream is idle. */
 int mpm_engine_synchronize(mpm_engine* h);
+
+/* Saves the particle state (positions, velocities, C, F, substep count) in
+ * device memory, and restores it, re-laid out for the current options (no
+ * reference counterpart: used by the autotuner to rewind its trials).  The
+ * grid needs no saving: each steps() call starts with a P2G of the particles. */
+int mpm_engine_snapshot(mpm_engine* h);
+int mpm_engine_restore(mpm_engine* h);
 
 /* ---- read-back ---- */
 

@@ -137,6 +137,14 @@ def load_library() -> ctypes.CDLL:
         "mpm_engine_debug_p2g": (ctypes.c_int, [eng, c_float_p]),
         "mpm_engine_debug_grid_update": (ctypes.c_int, [eng, c_float_p, c_float_p]),
         "mpm_engine_debug_g2p": (ctypes.c_int, [eng, c_float_p]),
+        "mpm_engine_snapshot": (ctypes.c_int, [eng]),
+        "mpm_engine_restore": (ctypes.c_int, [eng]),
+        "mpm_batch_create": (ctypes.c_int, [ctypes.POINTER(SimConfig), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+        "mpm_batch_init_particles": (ctypes.c_int, [eng, c_int_p, c_float_p]),
+        "mpm_batch_init_sdf_from_triangles": (ctypes.c_int, [eng, ctypes.c_int, c_float_p, ctypes.c_int]),
+        "mpm_batch_set_sdf": (ctypes.c_int, [eng, ctypes.c_int, c_float_p]),
+        "mpm_batch_get_grid": (ctypes.c_int, [eng, ctypes.c_int, c_float_p]),
+        "mpm_batch_scene_info": (ctypes.c_int, [eng, ctypes.c_int, c_i64_p]),
         "mpm_probe_copy": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, c_dbl_p]),
         "mpm_scene_fill": (
             ctypes.c_int,
@@ -271,10 +279,19 @@ class Engine:
     def __init__(self, cfg: SimConfig, **options):
         self._lib = load_library()
         self.cfg = cfg
+        self._nodes = cfg.nodes
         self._h = ctypes.c_void_p()
         _check(self._lib.mpm_engine_create(ctypes.byref(cfg), ctypes.byref(self._h)))
         for k, v in options.items():
             self.set_option(k, v)
+
+    def snapshot(self) -> None:
+        """Saves the particle state on the device (see restore)."""
+        _check(self._lib.mpm_engine_snapshot(self._h))
+
+    def restore(self) -> None:
+        """Returns to the last snapshot, laid out for the current options."""
+        _check(self._lib.mpm_engine_restore(self._h))
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
@@ -315,12 +332,12 @@ class Engine:
 
     def set_sdf(self, sdf) -> None:
         s = _f32(sdf).reshape(-1)
-        if s.size != self.cfg.nodes:
-            raise ValueError("sdf must have SIZE_X*SIZE_Y*SIZE_Z values")
+        if s.size != self._nodes:
+            raise ValueError("sdf must have one value per grid node")
         _check(self._lib.mpm_engine_set_sdf(self._h, _fptr(s)))
 
     def get_sdf(self) -> np.ndarray:
-        out = np.empty(self.cfg.nodes, dtype=np.float32)
+        out = np.empty(self._nodes, dtype=np.float32)
         _check(self._lib.mpm_engine_get_sdf(self._h, _fptr(out)))
         return out
 
@@ -350,7 +367,7 @@ class Engine:
         return out
 
     def getGrid(self) -> np.ndarray:
-        out = np.empty((4, self.cfg.nodes), dtype=np.float32)
+        out = np.empty((4, self._nodes), dtype=np.float32)
         _check(self._lib.mpm_engine_get_grid(self._h, _fptr(out)))
         return out
 
@@ -377,7 +394,7 @@ class Engine:
                 "claim_losers": int(out[3]), "rounds": int(out[4])}
 
     def debug_p2g(self) -> np.ndarray:
-        out = np.empty((4, self.cfg.nodes), dtype=np.float32)
+        out = np.empty((4, self._nodes), dtype=np.float32)
         _check(self._lib.mpm_engine_debug_p2g(self._h, _fptr(out)))
         return out
 
@@ -390,6 +407,62 @@ class Engine:
     def debug_g2p(self, grid4) -> None:
         g = _f32(grid4).reshape(4, -1)
         _check(self._lib.mpm_engine_debug_g2p(self._h, _fptr(g)))
+
+
+class Batch(Engine):
+    """Several independent scenes in one engine: every stage of a substep is
+    one launch over all scenes (no reference counterpart).
+
+    Particles of all scenes form one caller-order array, scene after scene;
+    ``init(counts, buffer)`` takes the per-scene counts and the (26, sum)
+    buffer.  The inherited read-back methods return that concatenation.
+    """
+
+    def __init__(self, cfgs, **options):
+        self._lib = load_library()
+        self.cfgs = list(cfgs)
+        arr = (SimConfig * len(self.cfgs))(*self.cfgs)
+        self.cfg = self.cfgs[0]
+        self._nodes = sum(c.nodes for c in self.cfgs)
+        self._h = ctypes.c_void_p()
+        _check(self._lib.mpm_batch_create(arr, len(self.cfgs), ctypes.byref(self._h)))
+        for k, v in options.items():
+            self.set_option(k, v)
+
+    @property
+    def scenes(self) -> int:
+        return len(self.cfgs)
+
+    def init(self, counts, buffer) -> None:
+        c = np.ascontiguousarray(counts, dtype=np.int32)
+        if c.size != self.scenes:
+            raise ValueError("one count per scene")
+        buf = _f32(buffer).reshape(-1)
+        if buf.size != PARTICLE_ATTRS * int(c.sum()):
+            raise ValueError("buffer must hold 26 * sum(counts) floats")
+        _check(self._lib.mpm_batch_init_particles(self._h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                                  _fptr(buf) if buf.size else None))
+
+    def initSDF(self, scene: int, triangles) -> None:
+        t = _f32(triangles).reshape(-1, 9)
+        _check(self._lib.mpm_batch_init_sdf_from_triangles(self._h, int(scene), _fptr(t) if t.size else None,
+                                                           t.shape[0]))
+
+    def set_scene_sdf(self, scene: int, sdf) -> None:
+        s = _f32(sdf).reshape(-1)
+        if s.size != self.cfgs[scene].nodes:
+            raise ValueError("sdf must have one value per node of the scene")
+        _check(self._lib.mpm_batch_set_sdf(self._h, int(scene), _fptr(s)))
+
+    def scene_info(self, scene: int) -> dict:
+        out = np.zeros(4, dtype=np.int64)
+        _check(self._lib.mpm_batch_scene_info(self._h, int(scene), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return {"first": int(out[0]), "count": int(out[1]), "node_offset": int(out[2]), "nodes": int(out[3])}
+
+    def scene_grid(self, scene: int) -> np.ndarray:
+        out = np.empty((4, self.cfgs[scene].nodes), dtype=np.float32)
+        _check(self._lib.mpm_batch_get_grid(self._h, int(scene), _fptr(out)))
+        return out
 
 
 def probe_copy(nbytes: int, iters: int = 20) -> float:

@@ -11,8 +11,11 @@
 
 namespace mpm {
 
-// Device-side copy of the scalar config.  Passed by value as a kernel argument
-// (the reference keeps it in __constant__ d_config, config.cu:3).
+// Device-side copy of one scene's scalar config (the reference keeps a single
+// SimConfig in __constant__ d_config, config.cu:3).  An engine holds one or
+// more independent scenes; their grids, flags and SDFs are concatenated, and
+// goff/foff locate the scene's part.  Kernels read the scene's DevCfg from a
+// device table indexed by the scene id of their work item.
 struct DevCfg {
     int sx, sy, sz;  // SIZE_X/Y/Z
     int sxy;         // SIZE_X*SIZE_Y
@@ -20,7 +23,27 @@ struct DevCfg {
     float gravity, dt, gamma, compression;
     float vol, mass;  // per-particle V = RESOLUTION^3 and m = 1 (initFMatrices)
     int fbx, fby, fbz;  // activity-flag blocks (4^3 nodes) per axis
+    int goff;           // first node of the scene in the concatenated grid
+    int foff;           // first flag block of the scene
 };
+
+// Scene lookup for per-particle kernels: particle (caller index) i belongs to
+// scene s with pstart[s] <= i < pstart[s + 1].
+struct SceneTable {
+    const DevCfg* cfgs;
+    const int* pstart;  // nscenes + 1 entries
+    int nscenes;
+    int local_bits;     // bits of a scene-local base-cell key; the scene id sits above
+};
+
+__device__ __forceinline__ int scene_of(const SceneTable& t, int i) {
+    int lo = 0, hi = t.nscenes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (t.pstart[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
 
 // Particle planes: attribute-major, attr*stride + slot.  Attributes 0..23 of
 // the reference layout (pos, vel, C, F); VOL and MASS are engine scalars.
@@ -45,7 +68,8 @@ struct ChunkDesc {
     int count;       // particles in the chunk
     int ox, oy, oz;  // node origin of the chunk's LDS tile
     int S;           // rounds (largest group)
-    int pad0, pad1;
+    int scene;       // scene of the chunk (index into the DevCfg table)
+    int pad1;
 };
 
 constexpr int BLOCK_CELLS = 4;  // chunk block edge, base cells
@@ -72,7 +96,8 @@ constexpr int TILE_PZ = MPM_TILE_PZ;
 constexpr int TILE_SLOTS = (TILE_D - 1) * (1 + TILE_PY + TILE_PZ) + 1;
 static_assert(TILE_PY >= TILE_D && TILE_PZ >= TILE_PY * TILE_D, "tile pitches must not overlap");
 
-__device__ __forceinline__ int gidx(const DevCfg& c, int x, int y, int z) { return z * c.sxy + y * c.sx + x; }
+// getGridIdx, common.cuh:131-134, offset to the scene's part of the grid.
+__device__ __forceinline__ int gidx(const DevCfg& c, int x, int y, int z) { return c.goff + z * c.sxy + y * c.sx + x; }
 
 // isInBounds, common.cuh:212-218 (negative coordinates compare as large unsigned).
 __device__ __forceinline__ bool inb(const DevCfg& c, int x, int y, int z) {
@@ -80,7 +105,7 @@ __device__ __forceinline__ bool inb(const DevCfg& c, int x, int y, int z) {
 }
 
 __device__ __forceinline__ int flag_block(const DevCfg& c, int x, int y, int z) {
-    return ((z >> 2) * c.fby + (y >> 2)) * c.fbx + (x >> 2);
+    return c.foff + ((z >> 2) * c.fby + (y >> 2)) * c.fbx + (x >> 2);
 }
 
 // spline, common.cuh:205-210.  `0.75 - x*x` rounds the float product first and

@@ -2,6 +2,9 @@
 //
 // Replaces the reference's Engine class (engine.h:14-43, engine.cu:36-114).
 // Differences of design (not of results):
+//  * an engine holds one or more independent scenes (each with its own
+//    SimConfig, grid, SDF and particles); every stage is one launch over all
+//    scenes, with per-scene configs in a device table;
 //  * the particle state lives in a chunked, Morton-sorted slot layout and is
 //    physically reordered at each re-sort (the reference only permutes an
 //    index array, engine.cu:98-99); the caller's order is kept in `orig`;
@@ -10,6 +13,7 @@
 //    fused kernel on two grid buffers;
 //  * the grid is cleared and updated only where the previous P2G wrote
 //    (4^3-node activity flags), instead of a dense pass over all nodes;
+//  * a k-substep call can be captured once as a HIP graph and replayed;
 //  * errors are returned as codes (the reference calls exit(1)).
 #include <hip/hip_runtime.h>
 
@@ -39,6 +43,8 @@ int fail(int code, const std::string& msg) {
 
 enum Stage { ST_P2G = 0, ST_G2P = 1, ST_G2P2G = 2, ST_GRID = 3, ST_SORT = 4, ST_COUNT = 5 };
 
+constexpr int MAX_SCENES = 4096;
+
 }  // namespace
 
 namespace mpm {
@@ -48,19 +54,31 @@ void set_last_error(const std::string& msg) { g_err = msg; }
 using namespace mpm;
 
 struct mpm_engine {
-    mpm_sim_config cfg{};
-    DevCfg dc{};
+    // ---- scenes ----
+    std::vector<mpm_sim_config> cfgs;  // host copies (derived fields filled)
+    std::vector<DevCfg> dcs;           // host copies of the device table
+    DevCfg* d_cfgs = nullptr;          // device table
+    uint16_t* d_block_scene = nullptr; // scene of each flag block
+    int S = 1;
+    std::vector<int> pstart;  // S + 1: particle ranges (caller order)
+    int* d_pstart = nullptr;
+    int local_bits = 0;  // bits of a scene-local base-cell key
+    int key_bits = 0;    // local_bits + scene bits
+
     int device = 0;
     hipStream_t stream = nullptr;
 
     int n = 0;
     bool have_particles = false;
     int chunk_cap = 512;
+    int launch_pct = 100;  // transfer workgroups per 100 chunks
     int resort_every = 1;
     bool timing = false;
+    bool use_graph = false;
+    bool capturing = false;
 
-    size_t G = 0;
-    int nfb = 0;
+    size_t G = 0;  // nodes of all scenes
+    int nfb = 0;   // flag blocks of all scenes
 
     float* planes[2] = {nullptr, nullptr};
     int* orig[2] = {nullptr, nullptr};
@@ -69,9 +87,9 @@ struct mpm_engine {
 
     ChunkDesc* chunks = nullptr;
     uint8_t* rounds = nullptr;  // particles per round of each chunk, at chunk.start + round
-    int chunk_limit = 0;  // bound on the chunk count (min(n, 4^3 blocks of the grid) + n / CHUNK + 1)
+    int chunk_limit = 0;  // bound on the chunk count (sum over scenes of min(n_s, blocks_s) + n / CHUNK + 1)
+    int chunk_alloc = 0;  // descriptors allocated
     int chunk_est = 0;    // chunk count at the last host synchronisation (sizes launches)
-    int key_bits = 0;     // bits of the engine's base-cell keys
 
     GridBuf grid[2] = {};
     int vcur = 0;  // grid[vcur]: velocities of the last update; grid[1 - vcur]: accumulates
@@ -86,6 +104,25 @@ struct mpm_engine {
     int64_t last_overflow = 0;
     int64_t last_claim_losers = 0;
     int64_t last_rounds = 0;
+
+    // Cached graph of one steps(k) call, valid for the start state it was
+    // captured from and the launch size it was captured with.
+    struct GraphEntry {
+        hipGraphExec_t exec = nullptr;
+        int k = 0;
+        int64_t phase = 0;  // substeps - sorted_at at the start
+        int vcur = 0, pcur = 0;
+        int launch = 0;     // transfer launch size baked in
+        // end state
+        int64_t end_phase = 0;
+        int end_vcur = 0, end_pcur = 0;
+    } graph;
+    int64_t graph_replays = 0;
+
+    // Saved particle state (mpm_engine_snapshot / mpm_engine_restore).
+    float* snap_planes = nullptr;  // A_STATE x n
+    int* snap_orig = nullptr;
+    int64_t snap_substeps = -1;
 
     struct Ev {
         hipEvent_t a, b;
@@ -115,13 +152,13 @@ struct StageTimer {
     int stage;
     hipEvent_t a = nullptr;
     StageTimer(mpm_engine* h_, int s) : h(h_), stage(s) {
-        if (h->timing) {
+        if (h->timing && !h->capturing) {
             a = take_event(h);
             hipEventRecord(a, h->stream);
         }
     }
     ~StageTimer() {
-        if (h->timing && a) {
+        if (a) {
             hipEvent_t b = take_event(h);
             hipEventRecord(b, h->stream);
             h->evs.push_back({a, b, stage});
@@ -142,7 +179,16 @@ template <class T> void dev_free(T*& p) {
     p = nullptr;
 }
 
+void drop_graph(mpm_engine* h) {
+    if (h->graph.exec) hipGraphExecDestroy(h->graph.exec);
+    h->graph = mpm_engine::GraphEntry{};
+}
+
 void free_particles(mpm_engine* h) {
+    drop_graph(h);
+    dev_free(h->snap_planes);
+    dev_free(h->snap_orig);
+    h->snap_substeps = -1;
     for (int i = 0; i < 2; ++i) {
         dev_free(h->planes[i]);
         dev_free(h->orig[i]);
@@ -150,6 +196,7 @@ void free_particles(mpm_engine* h) {
     }
     dev_free(h->chunks);
     dev_free(h->rounds);
+    dev_free(h->d_pstart);
     dev_free(h->sc.keys[0]);
     dev_free(h->sc.keys[1]);
     dev_free(h->sc.vals[0]);
@@ -161,6 +208,7 @@ void free_particles(mpm_engine* h) {
     dev_free(h->sc.seg_nch);
     h->sc.cap = h->sc.hist_cap = h->sc.partial_cap = 0;
     h->chunk_limit = 0;
+    h->chunk_alloc = 0;
     h->chunk_est = 0;
     h->have_particles = false;
     h->n = 0;
@@ -195,6 +243,31 @@ int ensure_planes(mpm_engine* h, int which, size_t slots) {
     return MPM_OK;
 }
 
+// Chunk-count bound for the current particles and chunk_cap: chunks are the
+// non-empty 4^3 blocks of every scene split into pieces of at most chunk_cap.
+int chunk_bound(const mpm_engine* h, int cap) {
+    int64_t b = 0;
+    for (int s = 0; s < h->S; ++s) {
+        const int64_t ns = h->pstart[s + 1] - h->pstart[s];
+        b += std::min<int64_t>(ns, (int64_t)h->dcs[s].fbx * h->dcs[s].fby * h->dcs[s].fbz);
+    }
+    return (int)(b + (h->n + cap - 1) / cap + 1);
+}
+
+// Sets chunk_cap and makes sure the chunk table can hold its bound.
+int set_chunk_cap(mpm_engine* h, int cap) {
+    const int lim = chunk_bound(h, cap);
+    if (lim > h->chunk_alloc) {
+        dev_free(h->chunks);
+        int rc;
+        if ((rc = dev_alloc(&h->chunks, (size_t)lim))) return rc;
+        h->chunk_alloc = lim;
+    }
+    h->chunk_cap = cap;
+    h->chunk_limit = lim;
+    return MPM_OK;
+}
+
 // Copies the 16 device counters (SortScratch::counters) into h->pinned and
 // waits for the stream.
 int read_counters(mpm_engine* h) {
@@ -203,14 +276,23 @@ int read_counters(mpm_engine* h) {
     return MPM_OK;
 }
 
-// Sorts sc.keys[0]/vals[0] (n base-cell keys), builds the chunk table and
-// gathers the particles from (src, src_stride, src_orig) into the alternate
-// plane buffer.  Nothing here waits for the device: the chunk count stays in
-// device memory and is bounded by chunk_limit by construction.
+SceneTable scene_table(const mpm_engine* h) {
+    SceneTable t;
+    t.cfgs = h->d_cfgs;
+    t.pstart = h->d_pstart;
+    t.nscenes = h->S;
+    t.local_bits = h->local_bits;
+    return t;
+}
+
+// Sorts sc.keys[0]/vals[0] (n scene/base-cell keys), builds the chunk table
+// and gathers the particles from (src, src_stride, src_orig) into the
+// alternate plane buffer.  Nothing here waits for the device: the chunk count
+// stays in device memory and is bounded by chunk_limit by construction.
 int sort_and_reorder(mpm_engine* h, const float* src, size_t src_stride, const int* src_orig, uint32_t mask) {
     const int n = h->n;
     const int r = radix_sort_pairs(h->sc, n, h->key_bits, h->stream);
-    build_chunks(h->sc, r, n, h->chunk_cap, h->chunks, h->chunk_limit, h->stream);
+    build_chunks(h->sc, r, n, h->chunk_cap, h->local_bits, h->chunks, h->chunk_limit, h->stream);
     const int dst = 1 - h->pcur;
     launch_reorder(h->chunks, h->sc.counters + 3, h->chunk_est, h->sc.keys[r], h->sc.vals[r], src, src_stride,
                    src_orig, h->planes[dst], h->plane_cap[dst], h->orig[dst], h->rounds, h->sc.counters + 4, mask,
@@ -227,14 +309,18 @@ int resort(mpm_engine* h, bool full) {
     StageTimer t(h, ST_SORT);
     const float* pl = h->planes[h->pcur];
     const size_t st = h->plane_cap[h->pcur];
-    launch_keys_linear(pl + A_PX * st, pl + A_PY * st, pl + A_PZ * st, h->n, true, h->dc, h->sc.keys[0],
-                       h->sc.vals[0], h->stream);
+    launch_keys_internal(pl + A_PX * st, pl + A_PY * st, pl + A_PZ * st, h->n, scene_table(h), h->orig[h->pcur],
+                         h->sc.keys[0], h->sc.vals[0], h->stream);
     return sort_and_reorder(h, h->planes[h->pcur], h->plane_cap[h->pcur], h->orig[h->pcur], full ? MASK_ALL : MASK_POS_F);
+}
+
+int transfer_launch_size(const mpm_engine* h) {
+    return std::max(1, (int)(((int64_t)h->chunk_est * h->launch_pct + 99) / 100));
 }
 
 TransferArgs make_args(mpm_engine* h) {
     TransferArgs a{};
-    a.c = h->dc;
+    a.cfgs = h->d_cfgs;
     a.part = h->planes[h->pcur];
     a.stride = h->plane_cap[h->pcur];
     a.chunks = h->chunks;
@@ -252,15 +338,16 @@ TransferArgs make_args(mpm_engine* h) {
     return a;
 }
 
-void run_transfer(mpm_engine* h, int mode) {
+void run_transfer(mpm_engine* h, int mode, int launch) {
     const int stage = mode == XFER_P2G ? ST_P2G : (mode == XFER_G2P ? ST_G2P : ST_G2P2G);
     StageTimer t(h, stage);
-    launch_transfer(mode, make_args(h), h->chunk_est, h->stream);
+    launch_transfer(mode, make_args(h), launch, h->stream);
 }
 
 void run_grid_update(mpm_engine* h) {
     StageTimer t(h, ST_GRID);
-    launch_grid_update(h->dc, h->grid[1 - h->vcur], h->grid[h->vcur], true, h->sdf, h->nfb, h->stream);
+    launch_grid_update(h->d_cfgs, h->d_block_scene, h->grid[1 - h->vcur], h->grid[h->vcur], true, h->sdf, h->nfb,
+                       h->stream);
     h->vcur = 1 - h->vcur;
 }
 
@@ -277,54 +364,153 @@ int check_engine(mpm_engine* h, bool need_particles) {
     return MPM_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-const char* mpm_last_error(void) { return g_err.c_str(); }
-
-int mpm_version(void) { return 100; }
-
-int mpm_config_derive(mpm_sim_config* cfg) {
-    if (!cfg) return fail(MPM_EINVAL, "null config");
-    const uint64_t g = (uint64_t)cfg->SIZE_X * cfg->SIZE_Y * cfg->SIZE_Z;
-    cfg->GRID_NUMBER = (uint32_t)g;
-    cfg->GRID_SIZE = (uint32_t)(g * 16u);
-    cfg->GRID_BLOCKS = (uint32_t)((g + 255u) / 256u);
+int check_scene(mpm_engine* h, int scene) {
+    if (scene < 0 || scene >= h->S) return fail(MPM_EINVAL, "scene index out of range");
     return MPM_OK;
 }
 
-int mpm_engine_create(const mpm_sim_config* cfg_in, mpm_engine** out) {
-    if (!cfg_in || !out) return fail(MPM_EINVAL, "null argument");
-    *out = nullptr;
-    mpm_sim_config cfg = *cfg_in;
-    if (cfg.SIZE_X == 0 || cfg.SIZE_Y == 0 || cfg.SIZE_Z == 0) return fail(MPM_EINVAL, "SIZE_X/Y/Z must be positive");
-    if (cfg.SIZE_X > 1024 || cfg.SIZE_Y > 1024 || cfg.SIZE_Z > 1024)
-        return fail(MPM_ELIMIT, "SIZE_X/Y/Z above 1024 (Morton keys cover 10 bits per axis)");
-    const uint64_t G = (uint64_t)cfg.SIZE_X * cfg.SIZE_Y * cfg.SIZE_Z;
-    if (G >= (1ull << 31)) return fail(MPM_ELIMIT, "grid has 2^31 or more nodes");
-    mpm_config_derive(&cfg);
-    mpm_engine* h = new mpm_engine();
-    h->cfg = cfg;
-    DevCfg& d = h->dc;
-    d.sx = (int)cfg.SIZE_X; d.sy = (int)cfg.SIZE_Y; d.sz = (int)cfg.SIZE_Z;
-    d.sxy = d.sx * d.sy;
-    d.pad = cfg.PADDING;
-    d.gravity = cfg.GRAVITY; d.dt = cfg.DT; d.gamma = cfg.GAMMA; d.compression = cfg.COMPRESSION;
-    d.vol = cfg.RESOLUTION * cfg.RESOLUTION * cfg.RESOLUTION;
-    d.mass = 1.0f;
-    d.fbx = (d.sx + 3) / 4; d.fby = (d.sy + 3) / 4; d.fbz = (d.sz + 3) / 4;
-    h->G = (size_t)G;
-    h->nfb = d.fbx * d.fby * d.fbz;
-    {
-        int maxdim = std::max(d.sx, std::max(d.sy, d.sz)), bits = 0;
-        while ((1 << bits) < maxdim) ++bits;
-        h->key_bits = 3 * std::max(bits, 1);
+// Enqueues k substeps (engine.cu:54-70 per substep) and advances the host-side
+// schedule state.  Launches only: safe under stream capture.
+int enqueue_steps(mpm_engine* h, int k, int launch) {
+    int rc;
+    launch_set_u32(h->overflow, 0, 0u, h->stream);
+    launch_set_u32(h->overflow, 1, 0u, h->stream);
+    for (int s = 0; s < k; ++s) {
+        if (s == 0) {
+            if (h->substeps - h->sorted_at >= h->resort_every) {
+                if ((rc = resort(h, true))) return rc;
+                h->sorted_at = h->substeps;
+            }
+            run_transfer(h, XFER_P2G, launch);
+        }
+        run_grid_update(h);
+        if (s == k - 1) {
+            run_transfer(h, XFER_G2P, launch);
+            h->substeps++;
+        } else {
+            run_transfer(h, XFER_G2P2G, launch);
+            h->substeps++;
+            if (h->substeps - h->sorted_at >= h->resort_every) {
+                if ((rc = resort(h, false))) return rc;
+                h->sorted_at = h->substeps;
+            }
+        }
     }
+    HIPCHK(hipGetLastError());
+    return MPM_OK;
+}
+
+// Replays the cached graph when it was captured from the same start state
+// (schedule phase, buffer parities, k) and a launch size still fitting the
+// chunk count; otherwise captures a new one.
+int steps_graph(mpm_engine* h, int k) {
+    mpm_engine::GraphEntry& g = h->graph;
+    const int64_t phase = h->substeps - h->sorted_at;
+    const int want = transfer_launch_size(h);
+    const bool hit = g.exec && g.k == k && g.phase == phase && g.vcur == h->vcur && g.pcur == h->pcur &&
+                     want <= g.launch && want * 2 >= g.launch;
+    if (!hit) {
+        drop_graph(h);
+        mpm_engine::GraphEntry e;
+        e.k = k;
+        e.phase = phase;
+        e.vcur = h->vcur;
+        e.pcur = h->pcur;
+        e.launch = want + want / 8 + 8;  // headroom for chunk-count growth
+        const int64_t sub0 = h->substeps, sorted0 = h->sorted_at;
+        HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        h->capturing = true;
+        int rc = enqueue_steps(h, k, e.launch);
+        h->capturing = false;
+        hipGraph_t graph = nullptr;
+        hipError_t ce = hipStreamEndCapture(h->stream, &graph);
+        if (rc || ce != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            h->substeps = sub0;
+            h->sorted_at = sorted0;
+            h->vcur = e.vcur;
+            h->pcur = e.pcur;
+            return rc ? rc : fail(MPM_EHIP, std::string("graph capture: ") + hipGetErrorString(ce));
+        }
+        hipError_t ie = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        e.end_phase = h->substeps - h->sorted_at;
+        e.end_vcur = h->vcur;
+        e.end_pcur = h->pcur;
+        h->substeps = sub0;  // the state advances when the graph runs
+        h->sorted_at = sorted0;
+        h->vcur = e.vcur;
+        h->pcur = e.pcur;
+        if (ie != hipSuccess) return fail(MPM_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+        g = e;
+    }
+    HIPCHK(hipGraphLaunch(g.exec, h->stream));
+    ++h->graph_replays;
+    h->substeps += k;
+    h->sorted_at = h->substeps - g.end_phase;
+    h->vcur = g.end_vcur;
+    h->pcur = g.end_pcur;
+    return MPM_OK;
+}
+
+int finish_steps(mpm_engine* h) {
+    int rc;
+    if ((rc = read_counters(h))) return rc;
+    h->last_overflow = h->pinned[8];
+    h->last_claim_losers = h->pinned[9];
+    h->chunk_est = (int)h->pinned[3];
+    h->last_rounds = h->pinned[4];
+    return MPM_OK;
+}
+
+int create_engine(const mpm_sim_config* cfgs_in, int S, mpm_engine** out) {
+    if (!cfgs_in || !out) return fail(MPM_EINVAL, "null argument");
+    *out = nullptr;
+    if (S < 1 || S > MAX_SCENES) return fail(MPM_ELIMIT, "scene count must be in [1, 4096]");
+    mpm_engine* h = new mpm_engine();
     auto cleanup = [&](int rc) {
         mpm_engine_destroy(h);
         return rc;
     };
+    h->S = S;
+    int64_t G = 0, nfb = 0;
+    int maxdim = 1;
+    for (int s = 0; s < S; ++s) {
+        mpm_sim_config cfg = cfgs_in[s];
+        if (cfg.SIZE_X == 0 || cfg.SIZE_Y == 0 || cfg.SIZE_Z == 0)
+            return cleanup(fail(MPM_EINVAL, "SIZE_X/Y/Z must be positive"));
+        if (cfg.SIZE_X > 1024 || cfg.SIZE_Y > 1024 || cfg.SIZE_Z > 1024)
+            return cleanup(fail(MPM_ELIMIT, "SIZE_X/Y/Z above 1024 (Morton keys cover 10 bits per axis)"));
+        mpm_config_derive(&cfg);
+        DevCfg d{};
+        d.sx = (int)cfg.SIZE_X; d.sy = (int)cfg.SIZE_Y; d.sz = (int)cfg.SIZE_Z;
+        d.sxy = d.sx * d.sy;
+        d.pad = cfg.PADDING;
+        d.gravity = cfg.GRAVITY; d.dt = cfg.DT; d.gamma = cfg.GAMMA; d.compression = cfg.COMPRESSION;
+        d.vol = cfg.RESOLUTION * cfg.RESOLUTION * cfg.RESOLUTION;
+        d.mass = 1.0f;
+        d.fbx = (d.sx + 3) / 4; d.fby = (d.sy + 3) / 4; d.fbz = (d.sz + 3) / 4;
+        const int64_t gs = (int64_t)d.sx * d.sy * d.sz;
+        if (G + gs >= (1ll << 31)) return cleanup(fail(MPM_ELIMIT, "scenes hold 2^31 or more grid nodes in all"));
+        d.goff = (int)G;
+        d.foff = (int)nfb;
+        G += gs;
+        nfb += (int64_t)d.fbx * d.fby * d.fbz;
+        maxdim = std::max(maxdim, std::max(d.sx, std::max(d.sy, d.sz)));
+        h->cfgs.push_back(cfg);
+        h->dcs.push_back(d);
+    }
+    {
+        int bits = 0, sbits = 0;
+        while ((1 << bits) < maxdim) ++bits;
+        while ((1 << sbits) < S) ++sbits;
+        h->local_bits = 3 * std::max(bits, 1);
+        h->key_bits = h->local_bits + sbits;
+        if (h->key_bits > 32)
+            return cleanup(fail(MPM_ELIMIT, "scene count and grid size need more than 32 key bits"));
+    }
+    h->G = (size_t)G;
+    h->nfb = (int)nfb;
     if (hipGetDevice(&h->device) != hipSuccess) return cleanup(fail(MPM_EHIP, "no HIP device"));
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(MPM_EHIP, "hipStreamCreate failed"));
@@ -338,15 +524,133 @@ int mpm_engine_create(const mpm_sim_config* cfg_in, mpm_engine** out) {
     }
     if ((rc = dev_alloc(&h->sdf, h->G))) return cleanup(rc);
     launch_fill_f32(h->sdf, h->G, 1e15f, h->stream);  // makeSDF with no triangles
+    if ((rc = dev_alloc(&h->d_cfgs, (size_t)S))) return cleanup(rc);
+    if ((rc = dev_alloc(&h->d_block_scene, (size_t)h->nfb))) return cleanup(rc);
+    {
+        std::vector<uint16_t> bs((size_t)h->nfb);
+        for (int s = 0; s < S; ++s) {
+            const DevCfg& d = h->dcs[s];
+            std::fill(bs.begin() + d.foff, bs.begin() + d.foff + (size_t)d.fbx * d.fby * d.fbz, (uint16_t)s);
+        }
+        if (hipMemcpy(h->d_block_scene, bs.data(), bs.size() * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(h->d_cfgs, h->dcs.data(), (size_t)S * sizeof(DevCfg), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(MPM_EHIP, "scene table upload failed"));
+    }
     if ((rc = dev_alloc(&h->sc.counters, 64))) return cleanup(rc);
     if (hipMemsetAsync(h->sc.counters, 0, 64 * sizeof(uint32_t), h->stream) != hipSuccess)
         return cleanup(fail(MPM_EHIP, "memset"));
     h->overflow = h->sc.counters + 8;
     if (hipHostMalloc((void**)&h->pinned, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
         return cleanup(fail(MPM_ENOMEM, "hipHostMalloc failed"));
+    h->pstart.assign((size_t)S + 1, 0);
     if (hipStreamSynchronize(h->stream) != hipSuccess) return cleanup(fail(MPM_EHIP, "stream sync"));
     *out = h;
     return MPM_OK;
+}
+
+// Loads counts[s] particles per scene from a 26 x sum(counts) SoA buffer
+// (scenes consecutive) and lays them out.
+int init_particles_impl(mpm_engine* h, const int* counts, const float* soa26) {
+    int64_t total = 0;
+    for (int s = 0; s < h->S; ++s) {
+        if (counts[s] < 0) return fail(MPM_EINVAL, "negative particle count");
+        total += counts[s];
+    }
+    if (total > (1ll << 30)) return fail(MPM_ELIMIT, "more than 2^30 particles");
+    const int n = (int)total;
+    if (n > 0 && !soa26) return fail(MPM_EINVAL, "null particle buffer");
+    free_particles(h);
+    int rc;
+    for (int b = 0; b < 2; ++b)
+        if ((rc = zero_grid(h, b))) return rc;
+    h->vcur = 0;
+    h->n = n;
+    h->substeps = 0;
+    h->sorted_at = 0;
+    h->pstart[0] = 0;
+    for (int s = 0; s < h->S; ++s) h->pstart[s + 1] = h->pstart[s] + counts[s];
+    if ((rc = dev_alloc(&h->d_pstart, (size_t)h->S + 1))) return rc;
+    HIPCHK(hipMemcpy(h->d_pstart, h->pstart.data(), sizeof(int) * (h->S + 1), hipMemcpyHostToDevice));
+    if (n == 0) {
+        h->have_particles = true;
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return MPM_OK;
+    }
+    float* tmp = nullptr;
+    if ((rc = dev_alloc(&tmp, (size_t)MPM_PARTICLE_ATTRS * n))) return rc;
+    HIPCHK(hipMemcpyAsync(tmp, soa26, sizeof(float) * MPM_PARTICLE_ATTRS * (size_t)n, hipMemcpyHostToDevice, h->stream));
+    launch_init_F(tmp, n, h->stream);
+    if ((rc = ensure_sort_scratch(h, (size_t)n)) || (rc = set_chunk_cap(h, h->chunk_cap)) ||
+        (rc = dev_alloc(&h->rounds, (size_t)n)) || (rc = ensure_planes(h, 0, (size_t)n)) ||
+        (rc = ensure_planes(h, 1, (size_t)n))) {
+        dev_free(tmp);
+        return rc;
+    }
+    h->chunk_est = h->chunk_limit;
+    launch_keys_internal(tmp, tmp + (size_t)n, tmp + 2 * (size_t)n, n, scene_table(h), nullptr, h->sc.keys[0],
+                         h->sc.vals[0], h->stream);
+    h->pcur = 1;  // sort_and_reorder writes into buffer 0
+    rc = sort_and_reorder(h, tmp, (size_t)n, nullptr, MASK_ALL);
+    hipStreamSynchronize(h->stream);
+    dev_free(tmp);
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
+    if ((rc = read_counters(h))) return rc;
+    h->chunk_est = (int)h->pinned[3];
+    h->last_rounds = h->pinned[4];
+    h->have_particles = true;
+    return MPM_OK;
+}
+
+int sdf_from_triangles(mpm_engine* h, int scene, const mpm_triangle* tris, int nt) {
+    if (nt < 0 || (nt > 0 && !tris)) return fail(MPM_EINVAL, "bad triangle buffer");
+    const DevCfg& d = h->dcs[scene];
+    if (nt == 0) {
+        launch_fill_f32(h->sdf + d.goff, (size_t)d.sxy * d.sz, 1e15f, h->stream);
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return MPM_OK;
+    }
+    float* dt = nullptr;
+    int rc;
+    if ((rc = dev_alloc(&dt, (size_t)9 * nt))) return rc;
+    HIPCHK(hipMemcpyAsync(dt, tris, sizeof(mpm_triangle) * (size_t)nt, hipMemcpyHostToDevice, h->stream));
+    launch_make_sdf(d, dt, nt, h->sdf, h->stream);
+    hipError_t e = hipStreamSynchronize(h->stream);
+    dev_free(dt);
+    if (e != hipSuccess) return fail(MPM_EHIP, std::string("makeSDF: ") + hipGetErrorString(e));
+    HIPCHK(hipGetLastError());
+    return MPM_OK;
+}
+
+int copy_grid_out(mpm_engine* h, int which, size_t off, size_t count, float* out, size_t out_stride) {
+    for (int p = 0; p < 4; ++p)
+        HIPCHK(hipMemcpyAsync(out + p * out_stride, h->grid[which].plane[p] + off, count * sizeof(float),
+                              hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return MPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mpm_last_error(void) { return g_err.c_str(); }
+
+int mpm_version(void) { return 110; }
+
+int mpm_config_derive(mpm_sim_config* cfg) {
+    if (!cfg) return fail(MPM_EINVAL, "null config");
+    const uint64_t g = (uint64_t)cfg->SIZE_X * cfg->SIZE_Y * cfg->SIZE_Z;
+    cfg->GRID_NUMBER = (uint32_t)g;
+    cfg->GRID_SIZE = (uint32_t)(g * 16u);
+    cfg->GRID_BLOCKS = (uint32_t)((g + 255u) / 256u);
+    return MPM_OK;
+}
+
+int mpm_engine_create(const mpm_sim_config* cfg, mpm_engine** out) { return create_engine(cfg, 1, out); }
+
+int mpm_batch_create(const mpm_sim_config* cfgs, int nscenes, mpm_engine** out) {
+    return create_engine(cfgs, nscenes, out);
 }
 
 int mpm_engine_destroy(mpm_engine* h) {
@@ -359,6 +663,8 @@ int mpm_engine_destroy(mpm_engine* h) {
         dev_free(h->grid[b].flags);
     }
     dev_free(h->sdf);
+    dev_free(h->d_cfgs);
+    dev_free(h->d_block_scene);
     dev_free(h->sc.counters);
     h->overflow = nullptr;
     if (h->pinned) hipHostFree(h->pinned);
@@ -381,11 +687,28 @@ int mpm_engine_set_option(mpm_engine* h, const char* key, int64_t value) {
     } else if (k == "CHUNK") {
         if (value < 64 || value > 4096 || (value % 64) != 0)
             return fail(MPM_EINVAL, "CHUNK must be a multiple of 64 in [64, 4096]");
-        if (h->have_particles && value != h->chunk_cap)
-            return fail(MPM_ESTATE, "CHUNK must be set before mpm_engine_init_particles");
-        h->chunk_cap = (int)value;
+        if (value == h->chunk_cap) return MPM_OK;
+        if (!h->have_particles || h->n == 0) {
+            h->chunk_cap = (int)value;
+            return MPM_OK;
+        }
+        // Re-lay the current particles out with the new chunk size.
+        int rc = check_engine(h, true);
+        if (rc) return rc;
+        drop_graph(h);
+        if ((rc = set_chunk_cap(h, (int)value)) || (rc = resort(h, true))) return rc;
+        h->sorted_at = h->substeps;
+        if ((rc = read_counters(h))) return rc;
+        h->chunk_est = (int)h->pinned[3];
+        h->last_rounds = h->pinned[4];
+    } else if (k == "LAUNCH_PCT") {
+        if (value < 1 || value > 100) return fail(MPM_EINVAL, "LAUNCH_PCT must be in [1, 100]");
+        h->launch_pct = (int)value;
     } else if (k == "TIMING") {
         h->timing = value != 0;
+    } else if (k == "GRAPH") {
+        h->use_graph = value != 0;
+        if (!h->use_graph) drop_graph(h);
     } else {
         return fail(MPM_EINVAL, "unknown option " + k);
     }
@@ -397,7 +720,11 @@ int mpm_engine_get_option(mpm_engine* h, const char* key, int64_t* value) {
     std::string k(key);
     if (k == "RESORT_EVERY") *value = h->resort_every;
     else if (k == "CHUNK") *value = h->chunk_cap;
+    else if (k == "LAUNCH_PCT") *value = h->launch_pct;
     else if (k == "TIMING") *value = h->timing ? 1 : 0;
+    else if (k == "GRAPH") *value = h->use_graph ? 1 : 0;
+    else if (k == "GRAPH_REPLAYS") *value = h->graph_replays;
+    else if (k == "SCENES") *value = h->S;
     else return fail(MPM_EINVAL, "unknown option " + k);
     return MPM_OK;
 }
@@ -405,62 +732,29 @@ int mpm_engine_get_option(mpm_engine* h, const char* key, int64_t* value) {
 int mpm_engine_init_particles(mpm_engine* h, int n, const float* soa26) {
     int rc = check_engine(h, false);
     if (rc) return rc;
+    if (h->S != 1) return fail(MPM_ESTATE, "engine holds several scenes: use mpm_batch_init_particles");
     if (n < 0) return fail(MPM_EINVAL, "negative particle count");
-    if (n > 0 && !soa26) return fail(MPM_EINVAL, "null particle buffer");
-    if ((int64_t)n > (1ll << 30)) return fail(MPM_ELIMIT, "more than 2^30 particles");
-    free_particles(h);
-    for (int b = 0; b < 2; ++b)
-        if ((rc = zero_grid(h, b))) return rc;
-    h->vcur = 0;
-    h->n = n;
-    h->substeps = 0;
-    h->sorted_at = 0;
-    if (n == 0) {
-        h->have_particles = true;
-        HIPCHK(hipStreamSynchronize(h->stream));
-        return MPM_OK;
-    }
-    float* tmp = nullptr;
-    if ((rc = dev_alloc(&tmp, (size_t)MPM_PARTICLE_ATTRS * n))) return rc;
-    HIPCHK(hipMemcpyAsync(tmp, soa26, sizeof(float) * MPM_PARTICLE_ATTRS * (size_t)n, hipMemcpyHostToDevice, h->stream));
-    launch_init_F(tmp, n, h->stream);
-    const int64_t blocks = (int64_t)h->dc.fbx * h->dc.fby * h->dc.fbz;
-    h->chunk_limit = (int)(std::min<int64_t>(n, blocks) + (n + h->chunk_cap - 1) / h->chunk_cap + 1);
-    h->chunk_est = h->chunk_limit;
-    if ((rc = ensure_sort_scratch(h, (size_t)n)) || (rc = dev_alloc(&h->chunks, (size_t)h->chunk_limit)) ||
-        (rc = dev_alloc(&h->rounds, (size_t)n)) ||
-        (rc = ensure_planes(h, 0, (size_t)n)) || (rc = ensure_planes(h, 1, (size_t)n))) {
-        dev_free(tmp);
-        return rc;
-    }
-    launch_keys_linear(tmp, tmp + (size_t)n, tmp + 2 * (size_t)n, n, true, h->dc, h->sc.keys[0], h->sc.vals[0],
-                       h->stream);
-    h->pcur = 1;  // sort_and_reorder writes into buffer 0
-    rc = sort_and_reorder(h, tmp, (size_t)n, nullptr, MASK_ALL);
-    hipStreamSynchronize(h->stream);
-    dev_free(tmp);
+    return init_particles_impl(h, &n, soa26);
+}
+
+int mpm_batch_init_particles(mpm_engine* h, const int* counts, const float* soa26) {
+    int rc = check_engine(h, false);
     if (rc) return rc;
-    HIPCHK(hipGetLastError());
-    if ((rc = read_counters(h))) return rc;
-    h->chunk_est = (int)h->pinned[3];
-    h->last_rounds = h->pinned[4];
-    h->have_particles = true;
-    return MPM_OK;
+    if (!counts) return fail(MPM_EINVAL, "null counts");
+    return init_particles_impl(h, counts, soa26);
 }
 
 int mpm_engine_init_sdf_from_triangles(mpm_engine* h, const mpm_triangle* tris, int nt) {
     int rc = check_engine(h, false);
     if (rc) return rc;
-    if (nt < 0 || (nt > 0 && !tris)) return fail(MPM_EINVAL, "bad triangle buffer");
-    float* dt = nullptr;
-    if ((rc = dev_alloc(&dt, (size_t)9 * std::max(nt, 1)))) return rc;
-    if (nt > 0) HIPCHK(hipMemcpyAsync(dt, tris, sizeof(mpm_triangle) * (size_t)nt, hipMemcpyHostToDevice, h->stream));
-    launch_make_sdf(h->dc, dt, nt, h->sdf, h->stream);
-    hipError_t e = hipStreamSynchronize(h->stream);
-    dev_free(dt);
-    if (e != hipSuccess) return fail(MPM_EHIP, std::string("makeSDF: ") + hipGetErrorString(e));
-    HIPCHK(hipGetLastError());
-    return MPM_OK;
+    if (h->S != 1) return fail(MPM_ESTATE, "engine holds several scenes: use mpm_batch_init_sdf_from_triangles");
+    return sdf_from_triangles(h, 0, tris, nt);
+}
+
+int mpm_batch_init_sdf_from_triangles(mpm_engine* h, int scene, const mpm_triangle* tris, int nt) {
+    int rc = check_engine(h, false);
+    if (rc || (rc = check_scene(h, scene))) return rc;
+    return sdf_from_triangles(h, scene, tris, nt);
 }
 
 int mpm_engine_set_sdf(mpm_engine* h, const float* sdf) {
@@ -468,6 +762,17 @@ int mpm_engine_set_sdf(mpm_engine* h, const float* sdf) {
     if (rc) return rc;
     if (!sdf) return fail(MPM_EINVAL, "null sdf");
     HIPCHK(hipMemcpyAsync(h->sdf, sdf, h->G * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return MPM_OK;
+}
+
+int mpm_batch_set_sdf(mpm_engine* h, int scene, const float* sdf) {
+    int rc = check_engine(h, false);
+    if (rc || (rc = check_scene(h, scene))) return rc;
+    if (!sdf) return fail(MPM_EINVAL, "null sdf");
+    const DevCfg& d = h->dcs[scene];
+    HIPCHK(hipMemcpyAsync(h->sdf + d.goff, sdf, (size_t)d.sxy * d.sz * sizeof(float), hipMemcpyHostToDevice,
+                          h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return MPM_OK;
 }
@@ -481,6 +786,18 @@ int mpm_engine_get_sdf(mpm_engine* h, float* out) {
     return MPM_OK;
 }
 
+int mpm_batch_scene_info(mpm_engine* h, int scene, int64_t* out4) {
+    if (!h || !out4) return fail(MPM_EINVAL, "null argument");
+    int rc = check_scene(h, scene);
+    if (rc) return rc;
+    const DevCfg& d = h->dcs[scene];
+    out4[0] = h->pstart[scene];
+    out4[1] = h->pstart[scene + 1] - h->pstart[scene];
+    out4[2] = d.goff;
+    out4[3] = (int64_t)d.sxy * d.sz;
+    return MPM_OK;
+}
+
 int mpm_engine_steps(mpm_engine* h, int k) {
     int rc = check_engine(h, true);
     if (rc) return rc;
@@ -489,38 +806,54 @@ int mpm_engine_steps(mpm_engine* h, int k) {
         h->substeps += k;
         return MPM_OK;
     }
-    HIPCHK(hipMemsetAsync(h->overflow, 0, 2 * sizeof(uint32_t), h->stream));
-    for (int s = 0; s < k; ++s) {
-        if (s == 0) {
-            if (h->substeps - h->sorted_at >= h->resort_every) {
-                if ((rc = resort(h, true))) return rc;
-                h->sorted_at = h->substeps;
-            }
-            run_transfer(h, XFER_P2G);
-        }
-        run_grid_update(h);
-        if (s == k - 1) {
-            run_transfer(h, XFER_G2P);
-            h->substeps++;
-        } else {
-            run_transfer(h, XFER_G2P2G);
-            h->substeps++;
-            if (h->substeps - h->sorted_at >= h->resort_every) {
-                if ((rc = resort(h, false))) return rc;
-                h->sorted_at = h->substeps;
-            }
-        }
+    if (h->use_graph) rc = steps_graph(h, k);
+    else rc = enqueue_steps(h, k, transfer_launch_size(h));
+    if (rc) return rc;
+    return finish_steps(h);
+}
+
+int mpm_engine_step(mpm_engine* h) { return mpm_engine_steps(h, 1); }
+
+int mpm_engine_snapshot(mpm_engine* h) {
+    int rc = check_engine(h, true);
+    if (rc) return rc;
+    if (h->n == 0) return MPM_OK;
+    if (!h->snap_planes) {
+        if ((rc = dev_alloc(&h->snap_planes, (size_t)A_STATE * h->n)) || (rc = dev_alloc(&h->snap_orig, (size_t)h->n)))
+            return rc;
     }
-    HIPCHK(hipGetLastError());
+    const size_t st = h->plane_cap[h->pcur];
+    for (int a = 0; a < A_STATE; ++a)
+        HIPCHK(hipMemcpyAsync(h->snap_planes + (size_t)a * h->n, h->planes[h->pcur] + (size_t)a * st,
+                              (size_t)h->n * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->snap_orig, h->orig[h->pcur], (size_t)h->n * sizeof(int), hipMemcpyDeviceToDevice,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->snap_substeps = h->substeps;
+    return MPM_OK;
+}
+
+int mpm_engine_restore(mpm_engine* h) {
+    int rc = check_engine(h, true);
+    if (rc) return rc;
+    if (h->n == 0) return MPM_OK;
+    if (h->snap_substeps < 0) return fail(MPM_ESTATE, "no snapshot taken");
+    drop_graph(h);
+    const size_t st = h->plane_cap[h->pcur];
+    for (int a = 0; a < A_STATE; ++a)
+        HIPCHK(hipMemcpyAsync(h->planes[h->pcur] + (size_t)a * st, h->snap_planes + (size_t)a * h->n,
+                              (size_t)h->n * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->orig[h->pcur], h->snap_orig, (size_t)h->n * sizeof(int), hipMemcpyDeviceToDevice,
+                          h->stream));
+    h->substeps = h->snap_substeps;
+    // The saved slots may belong to another chunk layout: lay them out anew.
+    if ((rc = resort(h, true))) return rc;
+    h->sorted_at = h->substeps;
     if ((rc = read_counters(h))) return rc;
-    h->last_overflow = h->pinned[8];
-    h->last_claim_losers = h->pinned[9];
     h->chunk_est = (int)h->pinned[3];
     h->last_rounds = h->pinned[4];
     return MPM_OK;
 }
-
-int mpm_engine_step(mpm_engine* h) { return mpm_engine_steps(h, 1); }
 
 int mpm_engine_synchronize(mpm_engine* h) {
     int rc = check_engine(h, false);
@@ -538,7 +871,7 @@ int mpm_engine_get_particles(mpm_engine* h, float* out) {
     if (h->n == 0) return MPM_OK;
     float* tmp = nullptr;
     if ((rc = dev_alloc(&tmp, (size_t)MPM_PARTICLE_ATTRS * h->n))) return rc;
-    launch_export_soa26(h->planes[h->pcur], h->plane_cap[h->pcur], h->orig[h->pcur], h->n, h->dc.vol, h->dc.mass, tmp,
+    launch_export_soa26(h->planes[h->pcur], h->plane_cap[h->pcur], h->orig[h->pcur], h->n, scene_table(h), tmp,
                         h->stream);
     hipError_t e = hipMemcpyAsync(out, tmp, sizeof(float) * MPM_PARTICLE_ATTRS * (size_t)h->n, hipMemcpyDeviceToHost,
                                   h->stream);
@@ -572,13 +905,14 @@ int mpm_engine_get_positions(mpm_engine* h, float* xyz) {
 int mpm_engine_get_sort(mpm_engine* h, int32_t* keys_out, int32_t* perm_out) {
     int rc = check_engine(h, true);
     if (rc) return rc;
+    if (h->S != 1) return fail(MPM_ESTATE, "get_sort is defined for single-scene engines");
     if (!keys_out || !perm_out) return fail(MPM_EINVAL, "null output");
     const int n = h->n;
     if (n == 0) return MPM_OK;
     float* tmp = nullptr;
     if ((rc = dev_alloc(&tmp, (size_t)3 * n))) return rc;
     launch_export_positions(h->planes[h->pcur], h->plane_cap[h->pcur], h->orig[h->pcur], n, tmp, h->stream);
-    launch_keys_linear(tmp, tmp + n, tmp + 2 * (size_t)n, n, false, h->dc, h->sc.keys[0], h->sc.vals[0], h->stream);
+    launch_keys_reference(tmp, tmp + n, tmp + 2 * (size_t)n, n, h->sc.keys[0], h->sc.vals[0], h->stream);
     const int r = radix_sort_pairs(h->sc, n, 0, h->stream);
     hipError_t e1 = hipMemcpyAsync(keys_out, h->sc.keys[r], sizeof(int32_t) * n, hipMemcpyDeviceToHost, h->stream);
     hipError_t e2 = hipMemcpyAsync(perm_out, h->sc.vals[r], sizeof(int32_t) * n, hipMemcpyDeviceToHost, h->stream);
@@ -593,11 +927,16 @@ int mpm_engine_get_grid(mpm_engine* h, float* out) {
     int rc = check_engine(h, false);
     if (rc) return rc;
     if (!out) return fail(MPM_EINVAL, "null output");
-    for (int p = 0; p < 4; ++p)
-        HIPCHK(hipMemcpyAsync(out + p * h->G, h->grid[h->vcur].plane[p], h->G * sizeof(float), hipMemcpyDeviceToHost,
-                              h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    return MPM_OK;
+    return copy_grid_out(h, h->vcur, 0, h->G, out, h->G);
+}
+
+int mpm_batch_get_grid(mpm_engine* h, int scene, float* out) {
+    int rc = check_engine(h, false);
+    if (rc || (rc = check_scene(h, scene))) return rc;
+    if (!out) return fail(MPM_EINVAL, "null output");
+    const DevCfg& d = h->dcs[scene];
+    const size_t gs = (size_t)d.sxy * d.sz;
+    return copy_grid_out(h, h->vcur, (size_t)d.goff, gs, out, gs);
 }
 
 int mpm_engine_get_stage_times(mpm_engine* h, double* out, int nstages) {
@@ -648,11 +987,8 @@ int mpm_engine_debug_p2g(mpm_engine* h, float* grid4_out) {
     if (rc) return rc;
     if (!grid4_out) return fail(MPM_EINVAL, "null output");
     const int m = 1 - h->vcur;
-    if (h->n > 0) run_transfer(h, XFER_P2G);
-    for (int p = 0; p < 4; ++p)
-        HIPCHK(hipMemcpyAsync(grid4_out + p * h->G, h->grid[m].plane[p], h->G * sizeof(float), hipMemcpyDeviceToHost,
-                              h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->n > 0) run_transfer(h, XFER_P2G, transfer_launch_size(h));
+    if ((rc = copy_grid_out(h, m, 0, h->G, grid4_out, h->G))) return rc;
     return zero_grid(h, m);
 }
 
@@ -664,10 +1000,8 @@ int mpm_engine_debug_grid_update(mpm_engine* h, const float* in, float* out) {
     for (int p = 0; p < 4; ++p)
         HIPCHK(hipMemcpyAsync(h->grid[m].plane[p], in + p * h->G, h->G * sizeof(float), hipMemcpyHostToDevice, h->stream));
     launch_fill_u8(h->grid[m].flags, (size_t)h->nfb, 1, h->stream);
-    launch_grid_update(h->dc, h->grid[m], h->grid[h->vcur], false, h->sdf, h->nfb, h->stream);
-    for (int p = 0; p < 4; ++p)
-        HIPCHK(hipMemcpyAsync(out + p * h->G, h->grid[m].plane[p], h->G * sizeof(float), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    launch_grid_update(h->d_cfgs, h->d_block_scene, h->grid[m], h->grid[h->vcur], false, h->sdf, h->nfb, h->stream);
+    if ((rc = copy_grid_out(h, m, 0, h->G, out, h->G))) return rc;
     return zero_grid(h, m);
 }
 
@@ -679,7 +1013,7 @@ int mpm_engine_debug_g2p(mpm_engine* h, const float* in) {
     for (int p = 0; p < 4; ++p)
         HIPCHK(hipMemcpyAsync(h->grid[v].plane[p], in + p * h->G, h->G * sizeof(float), hipMemcpyHostToDevice, h->stream));
     launch_fill_u8(h->grid[v].flags, (size_t)h->nfb, 1, h->stream);
-    if (h->n > 0) run_transfer(h, XFER_G2P);
+    if (h->n > 0) run_transfer(h, XFER_G2P, transfer_launch_size(h));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
     return MPM_OK;

@@ -21,7 +21,7 @@ struct GridBuf {
 enum TransferMode : int { XFER_P2G = 2, XFER_G2P = 1, XFER_G2P2G = 3 };
 
 struct TransferArgs {
-    DevCfg c;
+    const DevCfg* cfgs;   // per-scene configs (device table), indexed by ChunkDesc::scene
     float* part;          // particle planes, attr*stride + slot
     size_t stride;
     const ChunkDesc* chunks;
@@ -38,11 +38,13 @@ struct TransferArgs {
 // kernel loops until the device chunk count is covered.
 void launch_transfer(int mode, const TransferArgs& a, int grid_chunks, hipStream_t s);
 
-// Grid update over the 4^3 blocks flagged in `m`, zeroing the blocks flagged in
-// `v_clear` (the previous velocity buffer) and clearing those flags.
-void launch_grid_update(const DevCfg& c, GridBuf m, GridBuf v_clear, bool clear_v, const float* sdf, int nblocks,
-                        hipStream_t s);
+// Grid update over the 4^3 blocks flagged in `m` (blocks of all scenes,
+// block_scene[b] = scene of block b), zeroing the blocks flagged in `v_clear`
+// (the previous velocity buffer) and clearing those flags.
+void launch_grid_update(const DevCfg* cfgs, const uint16_t* block_scene, GridBuf m, GridBuf v_clear, bool clear_v,
+                        const float* sdf, int nblocks, hipStream_t s);
 
+// makeSDF of one scene (c: host copy of its DevCfg) into its part of `sdf`.
 void launch_make_sdf(const DevCfg& c, const float* tris, int nt, float* sdf, hipStream_t s);
 void launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
 void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s);
@@ -51,7 +53,7 @@ void launch_init_F(float* soa26, int n, hipStream_t s);
 
 // Export of the slot layout into the caller's order (slot s holds caller
 // particle orig[s]).
-void launch_export_soa26(const float* part, size_t stride, const int* orig, int n, float vol, float mass, float* out26,
+void launch_export_soa26(const float* part, size_t stride, const int* orig, int n, const SceneTable& t, float* out26,
                          hipStream_t s);
 void launch_export_positions(const float* part, size_t stride, const int* orig, int n, float* out_planes3,
                              hipStream_t s);
@@ -76,10 +78,17 @@ struct SortScratch {
 size_t radix_hist_elems(size_t n);
 size_t scan_partial_elems(size_t n);
 
-// Keys of position planes (vals = index): the engine's base-cell keys
-// (internal) or the reference's setKeys keys.
-void launch_keys_linear(const float* px, const float* py, const float* pz, int n, bool internal, const DevCfg& c,
-                        uint32_t* keys, uint32_t* vals, hipStream_t s);
+// p[i] = v as a one-thread kernel (usable under stream capture, unlike a
+// memset node).
+void launch_set_u32(uint32_t* p, int i, uint32_t v, hipStream_t s);
+
+// Keys of position planes (vals = index): the reference's setKeys keys, or
+// the engine's (scene, base-cell) keys (orig: caller index per slot, or null
+// for the identity).
+void launch_keys_reference(const float* px, const float* py, const float* pz, int n, uint32_t* keys, uint32_t* vals,
+                           hipStream_t s);
+void launch_keys_internal(const float* px, const float* py, const float* pz, int n, const SceneTable& t,
+                          const int* orig, uint32_t* keys, uint32_t* vals, hipStream_t s);
 // Stable LSD radix sort of (keys, vals) pairs; result in keys[r], vals[r] with
 // r returned (0 or 1).  With key_bits > 0 the keys are known to fit in that
 // many bits and only those digits are sorted; with key_bits = 0 the bits on
@@ -92,7 +101,8 @@ void scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* p
 // Splits the sorted base-cell keys into chunks (one 4^3 Morton block per
 // chunk, at most chunk_cap particles).  Writes up to chunk_limit descriptors; the chunk count
 // (possibly above the limit) lands in sc.counters[3].
-void build_chunks(SortScratch& sc, int r, int n, int chunk_cap, ChunkDesc* chunks, int chunk_limit, hipStream_t s);
+void build_chunks(SortScratch& sc, int r, int n, int chunk_cap, int local_bits, ChunkDesc* chunks, int chunk_limit,
+                  hipStream_t s);
 // Gathers the particles into the round layout of their chunks (one wave per
 // chunk; the chunk count is read on the device), writes the round sizes and
 // each chunk's round count, and adds the round counts into *rounds_total.

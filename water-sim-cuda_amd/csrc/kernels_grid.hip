@@ -10,15 +10,20 @@ namespace {
 // last P2G touched.  One wave per block (lane = node, x fastest).  The same
 // launch zeroes the blocks of the previous velocity buffer that its own P2G
 // had touched, so that buffer is all-zero again when it next accumulates.
-__global__ __launch_bounds__(256) void k_grid_update(DevCfg c, GridBuf m, GridBuf v, int clear_v,
-                                                     const float* __restrict__ sdf, int nblocks) {
-    const int blk = blockIdx.x * 4 + (threadIdx.x >> 6);
+// Blocks of all scenes are numbered consecutively; block_scene maps a block to
+// its scene.
+__global__ __launch_bounds__(256) void k_grid_update(const DevCfg* __restrict__ cfgs,
+                                                     const uint16_t* __restrict__ block_scene, GridBuf m, GridBuf v,
+                                                     int clear_v, const float* __restrict__ sdf, int nblocks) {
+    const int blk = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (blk >= nblocks) return;
     const int lane = threadIdx.x & 63;
     const bool fm = m.flags[blk] != 0;
     const bool fv = clear_v && v.flags[blk] != 0;
     if (!fm && !fv) return;
-    const int bx = blk % c.fbx, by = (blk / c.fbx) % c.fby, bz = blk / (c.fbx * c.fby);
+    const DevCfg c = cfgs[block_scene[blk]];
+    const int lb = blk - c.foff;
+    const int bx = lb % c.fbx, by = (lb / c.fbx) % c.fby, bz = lb / (c.fbx * c.fby);
     const int x = bx * 4 + (lane & 3), y = by * 4 + ((lane >> 2) & 3), z = bz * 4 + (lane >> 4);
     if (inb(c, x, y, z)) {
         const int g = gidx(c, x, y, z);
@@ -133,7 +138,7 @@ __global__ __launch_bounds__(256) void k_make_sdf(DevCfg c, const float* __restr
         if (d2 < mind) mind = d2;
     }
     wn = wn / (4.0f * 3.1415f);
-    sdf[idx] = (wn > 0.5f ? -1.0f : 1.0f) * sqrtf(mind);
+    sdf[c.goff + idx] = (wn > 0.5f ? -1.0f : 1.0f) * sqrtf(mind);
 }
 
 __global__ void k_fill_f32(float* p, size_t n, float v) {
@@ -152,14 +157,16 @@ __global__ void k_init_F(float* soa26, int n) {
     soa26[(size_t)A_F8 * n + i] = 1.0f;
 }
 
+// VOL and MASS come from the particle's scene (they are per-scene constants).
 __global__ __launch_bounds__(256) void k_export_soa26(const float* __restrict__ part, size_t stride,
-                                                      const int* __restrict__ orig, int n, float vol, float mass,
+                                                      const int* __restrict__ orig, int n, SceneTable t,
                                                       float* __restrict__ out) {
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
         const int id = orig[s];
         for (int a = 0; a < A_STATE; ++a) out[(size_t)a * n + id] = part[(size_t)a * stride + s];
-        out[(size_t)24 * n + id] = vol;
-        out[(size_t)25 * n + id] = mass;
+        const DevCfg& c = t.cfgs[scene_of(t, id)];
+        out[(size_t)24 * n + id] = c.vol;
+        out[(size_t)25 * n + id] = c.mass;
     }
 }
 
@@ -188,11 +195,11 @@ void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_copy, dim3(256 * 8 * 4), dim3(256), 0, s, src, dst, n);
 }
 
-void launch_grid_update(const DevCfg& c, GridBuf m, GridBuf v_clear, bool clear_v, const float* sdf, int nblocks,
-                        hipStream_t s) {
+void launch_grid_update(const DevCfg* cfgs, const uint16_t* block_scene, GridBuf m, GridBuf v_clear, bool clear_v,
+                        const float* sdf, int nblocks, hipStream_t s) {
     if (nblocks <= 0) return;
-    hipLaunchKernelGGL(k_grid_update, dim3((nblocks + 3) / 4), dim3(256), 0, s, c, m, v_clear, clear_v ? 1 : 0, sdf,
-                       nblocks);
+    hipLaunchKernelGGL(k_grid_update, dim3((nblocks + 3) / 4), dim3(256), 0, s, cfgs, block_scene, m, v_clear,
+                       clear_v ? 1 : 0, sdf, nblocks);
 }
 
 void launch_make_sdf(const DevCfg& c, const float* tris, int nt, float* sdf, hipStream_t s) {
@@ -219,11 +226,11 @@ void launch_init_F(float* soa26, int n, hipStream_t s) {
     hipLaunchKernelGGL(k_init_F, dim3((n + 255) / 256), dim3(256), 0, s, soa26, n);
 }
 
-void launch_export_soa26(const float* part, size_t stride, const int* orig, int n, float vol, float mass, float* out26,
+void launch_export_soa26(const float* part, size_t stride, const int* orig, int n, const SceneTable& t, float* out26,
                          hipStream_t s) {
     if (n <= 0) return;
     const int blocks = std::min((n + 255) / 256, 65536);
-    hipLaunchKernelGGL(k_export_soa26, dim3(blocks), dim3(256), 0, s, part, stride, orig, n, vol, mass, out26);
+    hipLaunchKernelGGL(k_export_soa26, dim3(blocks), dim3(256), 0, s, part, stride, orig, n, t, out26);
 }
 
 void launch_export_positions(const float* part, size_t stride, const int* orig, int n, float* out_planes3,

@@ -209,12 +209,26 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_apply(const uint32_t* in, u
 // ---- keys ----
 // Keys of position planes (vals = index): the engine's base-cell keys
 // (internal) or the reference's setKeys keys.
-__global__ void k_keys_linear(const float* __restrict__ px, const float* __restrict__ py,
-                              const float* __restrict__ pz, int n, int internal, DevCfg c, uint32_t* __restrict__ keys,
-                              uint32_t* __restrict__ vals) {
+// setKeys (kernels.cu:361-371): the reference's Morton keys of the positions.
+__global__ void k_keys_reference(const float* __restrict__ px, const float* __restrict__ py,
+                                 const float* __restrict__ pz, int n, uint32_t* __restrict__ keys,
+                                 uint32_t* __restrict__ vals) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    keys[i] = internal ? base_key(c, px[i], py[i], pz[i]) : sort_key(px[i], py[i], pz[i]);
+    keys[i] = sort_key(px[i], py[i], pz[i]);
+    vals[i] = (uint32_t)i;
+}
+
+// The engine's ordering key: scene id above the scene-local Morton code of the
+// base cell.  The scene comes from the caller index (orig, or the slot when
+// orig is null).
+__global__ void k_keys_internal(const float* __restrict__ px, const float* __restrict__ py,
+                                const float* __restrict__ pz, int n, SceneTable t, const int* __restrict__ orig,
+                                uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int sc = t.nscenes > 1 ? scene_of(t, orig ? orig[i] : i) : 0;
+    keys[i] = ((uint32_t)sc << t.local_bits) | base_key(t.cfgs[sc], px[i], py[i], pz[i]);
     vals[i] = (uint32_t)i;
 }
 
@@ -248,11 +262,13 @@ __global__ void k_seg_nch(const int* __restrict__ seg_start, const uint32_t* __r
 
 __global__ void k_chunk_desc(const int* __restrict__ seg_start, const uint32_t* __restrict__ counters, int n,
                              int cap, const uint32_t* __restrict__ first, const uint32_t* __restrict__ keys,
-                             ChunkDesc* __restrict__ chunks, int limit) {
+                             int local_bits, ChunkDesc* __restrict__ chunks, int limit) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n || s >= (int)counters[2]) return;
     const int st = seg_start[s], len = seg_start[s + 1] - st;
-    const uint32_t k = keys[st];  // base_key: Morton code of the base cell
+    const uint32_t kk = keys[st];
+    const int scene = (int)(local_bits >= 32 ? 0u : kk >> local_bits);
+    const uint32_t k = local_bits >= 32 ? kk : (kk & ((1u << local_bits) - 1u));  // Morton code of the base cell
     const int cx = (int)(compact_bits(k) & ~3u), cy = (int)(compact_bits(k >> 1) & ~3u),
               cz = (int)(compact_bits(k >> 2) & ~3u);
     int c = (int)first[s];
@@ -265,7 +281,8 @@ __global__ void k_chunk_desc(const int* __restrict__ seg_start, const uint32_t* 
         d.oy = cy - TILE_MARGIN;
         d.oz = cz - TILE_MARGIN;
         d.S = 0;  // set by the reorder
-        d.pad0 = d.pad1 = 0;
+        d.scene = scene;
+        d.pad1 = 0;
         chunks[c] = d;
     }
 }
@@ -344,11 +361,20 @@ void scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* p
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)tiles), dim3(SC_THREADS), 0, s, in, out, n, partial);
 }
 
-void launch_keys_linear(const float* px, const float* py, const float* pz, int n, bool internal, const DevCfg& c,
-                        uint32_t* keys, uint32_t* vals, hipStream_t s) {
+void launch_set_u32(uint32_t* p, int i, uint32_t v, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, p, i, v);
+}
+
+void launch_keys_reference(const float* px, const float* py, const float* pz, int n, uint32_t* keys, uint32_t* vals,
+                           hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_keys_linear, dim3((n + 255) / 256), dim3(256), 0, s, px, py, pz, n, internal ? 1 : 0, c, keys,
-                       vals);
+    hipLaunchKernelGGL(k_keys_reference, dim3((n + 255) / 256), dim3(256), 0, s, px, py, pz, n, keys, vals);
+}
+
+void launch_keys_internal(const float* px, const float* py, const float* pz, int n, const SceneTable& t,
+                          const int* orig, uint32_t* keys, uint32_t* vals, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_keys_internal, dim3((n + 255) / 256), dim3(256), 0, s, px, py, pz, n, t, orig, keys, vals);
 }
 
 int radix_sort_pairs(SortScratch& sc, int n, int key_bits, hipStream_t s) {
@@ -380,7 +406,8 @@ int radix_sort_pairs(SortScratch& sc, int n, int key_bits, hipStream_t s) {
     return r;
 }
 
-void build_chunks(SortScratch& sc, int r, int n, int chunk_cap, ChunkDesc* chunks, int chunk_limit, hipStream_t s) {
+void build_chunks(SortScratch& sc, int r, int n, int chunk_cap, int local_bits, ChunkDesc* chunks, int chunk_limit,
+                  hipStream_t s) {
     const unsigned blocks = (unsigned)((n + 255) / 256);
     uint32_t* heads = reinterpret_cast<uint32_t*>(sc.heads);
     uint32_t* hscan = sc.keys[1 - r];  // free after the sort
@@ -393,7 +420,7 @@ void build_chunks(SortScratch& sc, int r, int n, int chunk_cap, ChunkDesc* chunk
     hipLaunchKernelGGL(k_seg_nch, dim3(blocks), dim3(256), 0, s, sc.seg_start, sc.counters, n, chunk_cap, nch);
     scan_exclusive_u32(nch, first, (size_t)n, sc.partial, sc.counters + 3, s);
     hipLaunchKernelGGL(k_chunk_desc, dim3(blocks), dim3(256), 0, s, sc.seg_start, sc.counters, n, chunk_cap, first,
-                       sc.keys[r], chunks, chunk_limit);
+                       sc.keys[r], local_bits, chunks, chunk_limit);
 }
 
 void launch_reorder(ChunkDesc* ch, const uint32_t* nchunks_dev, int chunk_est, const uint32_t* keys_sorted,

@@ -286,7 +286,7 @@ __global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
     const int ci = xcd * per + t;
     if (ci >= (int)nch) break;
     const ChunkDesc d = a.chunks[ci];
-    const DevCfg& c = a.c;
+    const DevCfg c = a.cfgs[d.scene];
 
     __shared__ float4 vt[DO_G2P ? TILE_SLOTS : 1];
     __shared__ float4 ar[DO_P2G ? TILE_SLOTS : 1];  // mass, momentum x, y, z

@@ -90,4 +90,66 @@ def small_box(n_cells=(6, 6, 6), grid=32, physical: bool = True, seed: int = 7, 
                 resort_every=1, frames=10, description=f"{grid}^3 block {n_cells}")
 
 
+def sphere_sdf(cfg, centre, radius: float) -> np.ndarray:
+    """Analytic solid-sphere collider: node SDF = |x - centre| - radius."""
+    sz, sy, sx = cfg.grid_shape()
+    z, y, x = np.meshgrid(np.arange(sz), np.arange(sy), np.arange(sx), indexing="ij")
+    d = np.sqrt((x - centre[0]) ** 2 + (y - centre[1]) ** 2 + (z - centre[2]) ** 2) - radius
+    return d.astype(np.float32).reshape(-1)
+
+
+def batch_scenes(count: int = 64, seed: int = 4, n_min: int = 5000, n_max: int = 400000,
+                 edges=(32, 48, 64, 96), physical: bool = True) -> dict:
+    """Independent scenes for the batched engine.
+
+    Per scene (seeded): grid edge uniform in ``edges``; a target particle
+    count log-uniform in [n_min, n_max], capped at 0.3 * 8 * edge^3 so that it
+    fits the grid (a 400k-particle scene does not fit a 32^3 grid at 8 particles
+    per cell); a water sphere or cube of that volume at 8 ppc placed at random
+    in the upper half; a random collider, either a box floor (y below a random
+    height) or a solid sphere below the water, as an analytic node SDF.
+    DT 1e-3, 20 substeps per frame, PADDING 3.
+
+    Returns ``cfgs``, ``counts``, ``particles`` (26 x sum(counts), scene after
+    scene), ``sdfs`` (one array per scene) and ``description``.
+    """
+    m = _pkg()
+    rng = np.random.default_rng(seed)
+    cfgs, parts, sdfs, counts = [], [], [], []
+    for s in range(count):
+        e = int(rng.choice(edges))
+        cfg = m.make_config(SIZE_X=e, SIZE_Y=e, SIZE_Z=e, PADDING=3, DT=1e-3, RESOLUTION=1.0, SUBSTEPS=20,
+                            SDF_RESOLUTION=e, **_params(physical))
+        target = float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        target = min(target, 0.3 * 8 * e ** 3)
+        cells = target / 8.0
+        margin = 3 + 2
+        if rng.random() < 0.5:
+            r = (3.0 * cells / (4.0 * np.pi)) ** (1.0 / 3.0)
+            r = min(r, e / 2 - margin - 0.5)
+            lo_c, hi_c = margin + r, e - margin - r
+            c = [rng.uniform(lo_c, hi_c), rng.uniform(max(lo_c, e / 2 - 0.5 * r), hi_c), rng.uniform(lo_c, hi_c)]
+            p = m.scene_fill("sphere", c, (r,), seed * 1000 + s)
+        else:
+            a = min(cells ** (1.0 / 3.0), e - 2 * margin - 1)
+            lo_c, hi_c = margin, e - margin - a
+            lo = np.array([rng.uniform(lo_c, hi_c), rng.uniform(max(lo_c, min(e / 2 - a / 2, hi_c)), hi_c),
+                           rng.uniform(lo_c, hi_c)], dtype=np.float32)
+            lo = np.floor(lo)
+            p = m.scene_fill("box", lo, lo + np.float32(np.floor(a)), seed * 1000 + s)
+        if rng.random() < 0.5:
+            sdf = floor_sdf(cfg, float(rng.uniform(4.0, e / 4)))
+        else:
+            rc = float(rng.uniform(e / 8, e / 5))
+            sdf = sphere_sdf(cfg, (rng.uniform(e / 3, 2 * e / 3), 3.0 + 0.5 * rc, rng.uniform(e / 3, 2 * e / 3)), rc)
+        cfgs.append(cfg)
+        parts.append(p)
+        sdfs.append(sdf)
+        counts.append(p.shape[1])
+    return dict(name=f"batch{count}", cfgs=cfgs, counts=np.array(counts, dtype=np.int32),
+                particles=np.concatenate(parts, axis=1), sdfs=sdfs, resort_every=10, frames=100,
+                description=f"{count} scenes, edges {edges}, {n_min}-{n_max} particles log-uniform (capped at "
+                            f"0.3*8*edge^3), sphere/cube water, floor or sphere collider, DT 1e-3")
+
+
 SCENES = {"sphere64": sphere64, "dambreak128": dambreak128, "channel256": channel256}
