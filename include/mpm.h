@@ -177,6 +177,17 @@ int mpm_engine_synchronize(mpm_engine* h);
 int mpm_engine_snapshot(mpm_engine* h);
 int mpm_engine_restore(mpm_engine* h);
 
+/* Multiplies every scene's DT by `scale` (relative to its config) for the
+ * following steps (no reference counterpart: the learned corrector runs at
+ * 4x DT). */
+int mpm_engine_set_dt_scale(mpm_engine* h, float scale);
+
+/* Fallback checks of the current particles (no reference counterpart):
+ * out3 = {particles whose trilinear SDF (getSDF, common.cuh:136-167) is below
+ * -depth, mean |det F - 1|, particles with a non-finite position or
+ * velocity}. */
+int mpm_engine_check(mpm_engine* h, float depth, double* out3);
+
 /* ---- read-back ---- */
 
 /* Engine::getNumber (engine.h:31). */
@@ -253,6 +264,52 @@ int mpm_mesh_u_channel(const float* lo, const float* hi, float wall, float seg, 
                        int* ntris);
 /* Axis-aligned box mesh lo..hi (12 triangles, outward winding). */
 int mpm_mesh_box(const float* lo, const float* hi, mpm_triangle* tris_out, int* ntris);
+
+/* ---- learned grid-velocity corrector (no reference counterpart) ----
+ * An MLP 66 -> 64 -> 64 -> 64 -> 3 (ReLU; BF16 weights and activations, FP32
+ * accumulation on the matrix cores) maps per-node features after the grid
+ * update to a velocity correction that is added before G2P, so that a
+ * substep of k*DT approximates k plain substeps.  Parameters: W1[66][64],
+ * b1[64], W2[64][64], b2[64], W3[64][64], b3[64], W4[64][3], b4[3], FP32,
+ * W[in][out].  Feature rows have MPM_CORRECTOR_STRIDE floats (66 used, the
+ * rest zero); the layout is documented in csrc/corrector.hip. */
+#define MPM_CORRECTOR_FEATURES 66
+#define MPM_CORRECTOR_STRIDE 72
+#define MPM_CORRECTOR_PARAMS 12803
+typedef struct mpm_corrector mpm_corrector;
+
+int mpm_corrector_create(mpm_corrector** out);
+int mpm_corrector_destroy(mpm_corrector* c);
+int mpm_corrector_set_params(mpm_corrector* c, const float* params, int n);
+int mpm_corrector_get_params(mpm_corrector* c, float* params, int n);
+/* Input normalisation x' = (x - mean) * inv_std (66 each) and the output
+ * scale: correction = network output * out_scale. */
+int mpm_corrector_set_norm(mpm_corrector* c, const float* mean, const float* inv_std, float out_scale);
+/* Y[n][3] = corrections of the n feature rows X[n][72] (host arrays). */
+int mpm_corrector_forward(mpm_corrector* c, const float* X, int n, float* Y);
+/* Training set on the device: features X[n][72], target corrections T[n][3]. */
+int mpm_corrector_load_dataset(mpm_corrector* c, const float* X, const float* T, int n);
+/* Gradient (MPM_CORRECTOR_PARAMS floats) and mean loss of rows
+ * [row0, row0 + n) (wrapping) of the loaded set, loss = mean over rows and
+ * outputs of (output - target / out_scale)^2; no update. */
+int mpm_corrector_gradient(mpm_corrector* c, int row0, int n, float* grad_out, float* loss_out);
+/* `steps` Adam steps (lr, beta 0.9/0.999) over consecutive batches of the
+ * loaded set; loss_out (nullable) gets each step's batch loss. */
+int mpm_corrector_train(mpm_corrector* c, int steps, int batch, float lr, float* loss_out);
+
+/* Attaches a corrector (null detaches); option "CORRECT" = 1 applies it after
+ * every grid update. */
+int mpm_engine_set_corrector(mpm_engine* h, mpm_corrector* c);
+/* Training pairs at the current state: features of the active nodes of one
+ * substep at ratio*DT (after its grid update) and, as targets, the mean grid
+ * velocity of the same nodes over `ratio` plain substeps minus the coarse
+ * velocity (nodes with mass in every plain substep only).  The engine ends
+ * `ratio` plain substeps later.  Writes at most cap rows. */
+int mpm_engine_collect_pairs(mpm_engine* h, int ratio, float* X_out, float* T_out, int cap, int* count_out);
+/* Stage access for parity tests: the active nodes of the current grid
+ * (nodes_out[2i] = x | y << 10 | z << 20, nodes_out[2i+1] = scene) and their
+ * feature rows, at most cap. */
+int mpm_engine_debug_features(mpm_engine* h, int32_t* nodes_out, float* X_out, int cap, int* count_out);
 
 #ifdef __cplusplus
 }

@@ -137,6 +137,20 @@ def load_library() -> ctypes.CDLL:
         "mpm_engine_debug_p2g": (ctypes.c_int, [eng, c_float_p]),
         "mpm_engine_debug_grid_update": (ctypes.c_int, [eng, c_float_p, c_float_p]),
         "mpm_engine_debug_g2p": (ctypes.c_int, [eng, c_float_p]),
+        "mpm_engine_set_dt_scale": (ctypes.c_int, [eng, ctypes.c_float]),
+        "mpm_engine_check": (ctypes.c_int, [eng, ctypes.c_float, c_dbl_p]),
+        "mpm_engine_set_corrector": (ctypes.c_int, [eng, ctypes.c_void_p]),
+        "mpm_engine_collect_pairs": (ctypes.c_int, [eng, ctypes.c_int, c_float_p, c_float_p, ctypes.c_int, c_int_p]),
+        "mpm_engine_debug_features": (ctypes.c_int, [eng, c_i32_p, c_float_p, ctypes.c_int, c_int_p]),
+        "mpm_corrector_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+        "mpm_corrector_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+        "mpm_corrector_set_params": (ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int]),
+        "mpm_corrector_get_params": (ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int]),
+        "mpm_corrector_set_norm": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, ctypes.c_float]),
+        "mpm_corrector_forward": (ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_float_p]),
+        "mpm_corrector_load_dataset": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, ctypes.c_int]),
+        "mpm_corrector_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_float_p, c_float_p]),
+        "mpm_corrector_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float, c_float_p]),
         "mpm_engine_snapshot": (ctypes.c_int, [eng]),
         "mpm_engine_restore": (ctypes.c_int, [eng]),
         "mpm_batch_create": (ctypes.c_int, [ctypes.POINTER(SimConfig), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
@@ -265,7 +279,11 @@ def mesh_box(lo, hi) -> np.ndarray:
 
 # ---- engine ----
 
-STAGES = ("p2g", "g2p", "g2p2g", "grid_update", "resort")
+STAGES = ("p2g", "g2p", "g2p2g", "grid_update", "resort", "corrector")
+
+CORRECTOR_FEATURES = 66
+CORRECTOR_STRIDE = 72
+CORRECTOR_PARAMS = 12803
 
 
 class Engine:
@@ -284,6 +302,40 @@ class Engine:
         _check(self._lib.mpm_engine_create(ctypes.byref(cfg), ctypes.byref(self._h)))
         for k, v in options.items():
             self.set_option(k, v)
+
+    def set_dt_scale(self, scale: float) -> None:
+        """Runs the following substeps at scale * DT (every scene)."""
+        _check(self._lib.mpm_engine_set_dt_scale(self._h, float(scale)))
+
+    def check(self, depth: float = 0.5) -> dict:
+        """Fallback checks: particles deeper than `depth` in the collider,
+        mean |det F - 1|, non-finite particles."""
+        out = np.zeros(3, dtype=np.float64)
+        _check(self._lib.mpm_engine_check(self._h, float(depth), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return {"penetrating": int(out[0]), "mean_abs_J_minus_1": float(out[1]), "nonfinite": int(out[2])}
+
+    def set_corrector(self, corr) -> None:
+        self._corr = corr  # keeps the corrector alive while attached
+        _check(self._lib.mpm_engine_set_corrector(self._h, corr._h if corr is not None else None))
+
+    def collect_pairs(self, ratio: int, cap: int):
+        """(features (m, 72), targets (m, 3)) of one ratio*DT substep against
+        `ratio` plain substeps; advances the engine by `ratio` plain substeps."""
+        X = np.empty((cap, CORRECTOR_STRIDE), dtype=np.float32)
+        T = np.empty((cap, 3), dtype=np.float32)
+        cnt = ctypes.c_int(0)
+        _check(self._lib.mpm_engine_collect_pairs(self._h, int(ratio), _fptr(X), _fptr(T), int(cap), ctypes.byref(cnt)))
+        return X[:cnt.value].copy(), T[:cnt.value].copy()
+
+    def debug_features(self, cap: int):
+        """(nodes (m, 2) [x | y<<10 | z<<20, scene], features (m, 72)) of the
+        active nodes of the current grid."""
+        nodes = np.empty((cap, 2), dtype=np.int32)
+        X = np.empty((cap, CORRECTOR_STRIDE), dtype=np.float32)
+        cnt = ctypes.c_int(0)
+        _check(self._lib.mpm_engine_debug_features(self._h, nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                                   _fptr(X), int(cap), ctypes.byref(cnt)))
+        return nodes[:cnt.value].copy(), X[:cnt.value].copy()
 
     def snapshot(self) -> None:
         """Saves the particle state on the device (see restore)."""
@@ -462,6 +514,75 @@ class Batch(Engine):
     def scene_grid(self, scene: int) -> np.ndarray:
         out = np.empty((4, self.cfgs[scene].nodes), dtype=np.float32)
         _check(self._lib.mpm_batch_get_grid(self._h, int(scene), _fptr(out)))
+        return out
+
+
+class Corrector:
+    """The learned grid-velocity corrector's network on the device (no
+    reference counterpart): parameters, input normalisation, forward pass,
+    gradient and Adam training with the library's own kernels."""
+
+    def __init__(self, params=None):
+        self._lib = load_library()
+        self._h = ctypes.c_void_p()
+        _check(self._lib.mpm_corrector_create(ctypes.byref(self._h)))
+        if params is not None:
+            self.set_params(params)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.mpm_corrector_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params) -> None:
+        p = _f32(params).reshape(-1)
+        _check(self._lib.mpm_corrector_set_params(self._h, _fptr(p), p.size))
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(CORRECTOR_PARAMS, dtype=np.float32)
+        _check(self._lib.mpm_corrector_get_params(self._h, _fptr(p), p.size))
+        return p
+
+    def set_norm(self, mean, inv_std, out_scale: float) -> None:
+        m = _f32(mean, (CORRECTOR_FEATURES,))
+        s = _f32(inv_std, (CORRECTOR_FEATURES,))
+        _check(self._lib.mpm_corrector_set_norm(self._h, _fptr(m), _fptr(s), float(out_scale)))
+
+    @staticmethod
+    def _rows(X) -> np.ndarray:
+        X = _f32(X)
+        if X.ndim != 2 or X.shape[1] not in (CORRECTOR_FEATURES, CORRECTOR_STRIDE):
+            raise ValueError("features must be (n, 66) or (n, 72)")
+        if X.shape[1] == CORRECTOR_FEATURES:
+            X = np.concatenate([X, np.zeros((X.shape[0], CORRECTOR_STRIDE - CORRECTOR_FEATURES), np.float32)], 1)
+        return np.ascontiguousarray(X)
+
+    def forward(self, X) -> np.ndarray:
+        X = self._rows(X)
+        Y = np.empty((X.shape[0], 3), dtype=np.float32)
+        _check(self._lib.mpm_corrector_forward(self._h, _fptr(X), X.shape[0], _fptr(Y)))
+        return Y
+
+    def load_dataset(self, X, T) -> None:
+        X = self._rows(X)
+        T = _f32(T, (X.shape[0], 3))
+        _check(self._lib.mpm_corrector_load_dataset(self._h, _fptr(X), _fptr(T), X.shape[0]))
+
+    def gradient(self, row0: int, n: int):
+        g = np.empty(CORRECTOR_PARAMS, dtype=np.float32)
+        loss = ctypes.c_float(0.0)
+        _check(self._lib.mpm_corrector_gradient(self._h, int(row0), int(n), _fptr(g), ctypes.byref(loss)))
+        return g, float(loss.value)
+
+    def train(self, steps: int, batch: int, lr: float) -> np.ndarray:
+        out = np.empty(steps, dtype=np.float32)
+        _check(self._lib.mpm_corrector_train(self._h, int(steps), int(batch), float(lr), _fptr(out)))
         return out
 
 

@@ -35,7 +35,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 
-HIP_SOURCES = ["kernels_transfer.hip", "kernels_grid.hip", "kernels_sort.hip", "engine.hip"]
+HIP_SOURCES = ["kernels_transfer.hip", "kernels_grid.hip", "kernels_sort.hip", "engine.hip", "corrector.hip"]
 CXX_SOURCES = ["config.cpp", "scene.cpp"]
 
 

@@ -1,0 +1,803 @@
+// corrector.hip -- learned grid-velocity corrector (no reference counterpart).
+//
+// A small MLP (66 -> 64 -> 64 -> 64 -> 3, ReLU) maps per-node features of the
+// grid after the grid update to a velocity correction, so that a substep of
+// 4x the DT behaves like four plain substeps.  Weights and activations are
+// BF16, products accumulate in FP32 on the matrix cores
+// (v_mfma_f32_32x32x8_bf16).  Forward, backward and the Adam update are this
+// file's kernels; the CPU oracle (oracle/mlp.py) checks them.
+//
+// Matrix layout.  Every layer computes Out^T = W^T In^T with the nodes of a
+// 32-node tile as the MFMA's N dimension: A = W^T (32 output units x 8 inputs),
+// B = In^T (8 inputs x 32 nodes), D = Out^T (32 units x 32 nodes).  For the
+// 32x32x8 instruction lane l holds A[l%32][4(l/32)+i], B[4(l/32)+i][l%32] and
+// D[8(r/4) + 4(l/32) + r%4][l%32] (i = 0..3, r = 0..15), so the D registers
+// of one layer, grouped by four, are exactly the B operands of the next: the
+// forward pass moves no data between layers.  The weight gradients
+// dW = In Out-grad^T contract over nodes and go through two small LDS
+// transposes per layer.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+#include "mpm.h"
+
+namespace mpm {
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int NF = CORR_FEATURES;  // 66 features
+constexpr int KIN = CORR_STRIDE;   // 72: features padded to 9 K-blocks of 8
+constexpr int NH = 64;
+constexpr int NO = 3;
+
+constexpr int P_W1 = 0;                 // W1[66][64] (input-major)
+constexpr int P_B1 = P_W1 + NF * NH;
+constexpr int P_W2 = P_B1 + NH;         // W2[64][64]
+constexpr int P_B2 = P_W2 + NH * NH;
+constexpr int P_W3 = P_B2 + NH;         // W3[64][64]
+constexpr int P_B3 = P_W3 + NH * NH;
+constexpr int P_W4 = P_B3 + NH;         // W4[64][3]
+constexpr int P_B4 = P_W4 + NH * NO;
+static_assert(P_B4 + NO == CORR_PARAMS, "parameter count");
+
+// Fragments (64 lanes x 4 bf16 each).  Forward: A = W^T, blocks (out mb, in kb).
+constexpr int F_L1 = 0;    // 2 x 9
+constexpr int F_L2 = 18;   // 2 x 8
+constexpr int F_L3 = 34;   // 2 x 8
+constexpr int F_L4 = 50;   // 1 x 8 (outputs padded to 32)
+// Backward: A = W, blocks (in mb, out kb).
+constexpr int B_L4 = 58;   // 2 x 1 (outputs padded to 8)
+constexpr int B_L3 = 60;   // 2 x 8
+constexpr int B_L2 = 76;   // 2 x 8
+constexpr int NFRAG = 92;
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return 0x7FC0u;  // NaN
+    u += 0x7FFFu + ((u >> 16) & 1u);                       // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
+    bf16x4 r;
+    r[0] = (short)f2bf(a);
+    r[1] = (short)f2bf(b);
+    r[2] = (short)f2bf(c);
+    r[3] = (short)f2bf(d);
+    return r;
+}
+
+__device__ __forceinline__ f32x16 mfma(bf16x4 a, bf16x4 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Weight W[in][out] of layer L (1..4) with zero padding.
+__device__ __forceinline__ float w_at(const float* __restrict__ P, int L, int in, int out) {
+    switch (L) {
+        case 1: return (in < NF && out < NH) ? P[P_W1 + in * NH + out] : 0.0f;
+        case 2: return (in < NH && out < NH) ? P[P_W2 + in * NH + out] : 0.0f;
+        case 3: return (in < NH && out < NH) ? P[P_W3 + in * NH + out] : 0.0f;
+        default: return (in < NH && out < NO) ? P[P_W4 + in * NO + out] : 0.0f;
+    }
+}
+
+// One thread per (fragment, lane): converts the FP32 master weights into the
+// BF16 fragment order the MFMA operands read.
+__global__ void k_pack(const float* __restrict__ P, bf16x4* __restrict__ frags) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= NFRAG * 64) return;
+    const int f = t >> 6, l = t & 63, hi = l >> 5, lo = l & 31;
+    int L, mb, kb;
+    bool fwd = true;
+    if (f < F_L2) { L = 1; mb = f / 9; kb = f % 9; }
+    else if (f < F_L3) { L = 2; mb = (f - F_L2) / 8; kb = (f - F_L2) % 8; }
+    else if (f < F_L4) { L = 3; mb = (f - F_L3) / 8; kb = (f - F_L3) % 8; }
+    else if (f < B_L4) { L = 4; mb = 0; kb = f - F_L4; }
+    else if (f < B_L3) { L = 4; mb = f - B_L4; kb = 0; fwd = false; }
+    else if (f < B_L2) { L = 3; mb = (f - B_L3) / 8; kb = (f - B_L3) % 8; fwd = false; }
+    else { L = 2; mb = (f - B_L2) / 8; kb = (f - B_L2) % 8; fwd = false; }
+    float v[4];
+    for (int i = 0; i < 4; ++i) {
+        const int k = kb * 8 + 4 * hi + i, m = mb * 32 + lo;
+        v[i] = fwd ? w_at(P, L, k, m) : w_at(P, L, m, k);
+    }
+    frags[t] = pack4(v[0], v[1], v[2], v[3]);
+}
+
+// Bias of the D-layout element r of M-block mb.
+__device__ __forceinline__ f32x16 bias_acc(const float* __restrict__ b, int mb, int hi, int nvalid) {
+    f32x16 a;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int u = mb * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+        a[r] = u < nvalid ? b[u] : 0.0f;
+    }
+    return a;
+}
+
+__device__ __forceinline__ void relu_pack(const f32x16 a[2], bf16x4 h[8]) {
+#pragma unroll
+    for (int kb = 0; kb < 8; ++kb) {
+        const int mb = kb >> 2, q = (kb & 3) * 4;
+        h[kb] = pack4(fmaxf(a[mb][q], 0.0f), fmaxf(a[mb][q + 1], 0.0f), fmaxf(a[mb][q + 2], 0.0f),
+                      fmaxf(a[mb][q + 3], 0.0f));
+    }
+}
+
+// Forward pass of one 32-node tile.  xb: input B fragments (9 K-blocks).
+__device__ __forceinline__ void mlp_fwd(const bf16x4* __restrict__ fr, const float* __restrict__ P,
+                                        const bf16x4 xb[9], int lane, bf16x4 h1[8], bf16x4 h2[8], bf16x4 h3[8],
+                                        f32x16& a4) {
+    const int hi = lane >> 5;
+    f32x16 a[2];
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+        a[mb] = bias_acc(P + P_B1, mb, hi, NH);
+#pragma unroll
+        for (int kb = 0; kb < 9; ++kb) a[mb] = mfma(fr[(F_L1 + mb * 9 + kb) * 64 + lane], xb[kb], a[mb]);
+    }
+    relu_pack(a, h1);
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+        a[mb] = bias_acc(P + P_B2, mb, hi, NH);
+#pragma unroll
+        for (int kb = 0; kb < 8; ++kb) a[mb] = mfma(fr[(F_L2 + mb * 8 + kb) * 64 + lane], h1[kb], a[mb]);
+    }
+    relu_pack(a, h2);
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+        a[mb] = bias_acc(P + P_B3, mb, hi, NH);
+#pragma unroll
+        for (int kb = 0; kb < 8; ++kb) a[mb] = mfma(fr[(F_L3 + mb * 8 + kb) * 64 + lane], h2[kb], a[mb]);
+    }
+    relu_pack(a, h3);
+    a4 = bias_acc(P + P_B4, 0, hi, NO);
+#pragma unroll
+    for (int kb = 0; kb < 8; ++kb) a4 = mfma(fr[(F_L4 + kb) * 64 + lane], h3[kb], a4);
+}
+
+// Input fragments of the tile's node `row` (lanes l and l+32 hold the two
+// halves of each K-block): normalised features, zero beyond n.
+__device__ __forceinline__ void load_x(const float* __restrict__ X, int row, bool valid, const float* __restrict__ mean,
+                                       const float* __restrict__ inv_std, int hi, bf16x4 xb[9]) {
+#pragma unroll
+    for (int kb = 0; kb < 9; ++kb) {
+        const int f = kb * 8 + 4 * hi;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (valid) v = *reinterpret_cast<const float4*>(X + (size_t)row * KIN + f);
+        xb[kb] = pack4((v.x - mean[f]) * inv_std[f], (v.y - mean[f + 1]) * inv_std[f + 1],
+                       (v.z - mean[f + 2]) * inv_std[f + 2], (v.w - mean[f + 3]) * inv_std[f + 3]);
+    }
+}
+
+struct Norm {
+    const float* mean;     // KIN
+    const float* inv_std;  // KIN (0 for padding)
+    float out_scale;       // correction = network output * out_scale
+};
+
+// Forward over rows [0, *count) (or n when count is null): Y[row][3] =
+// out_scale * MLP(X[row]).  One wave per tile, persistent over tiles.
+__global__ __launch_bounds__(64) void k_forward(const float* __restrict__ X, int n, const uint32_t* __restrict__ count,
+                                                Norm nm, const bf16x4* __restrict__ fr, const float* __restrict__ P,
+                                                float* __restrict__ Y) {
+    const int rows = count ? (int)min((uint32_t)n, *count) : n;
+    const int lane = threadIdx.x, hi = lane >> 5;
+    for (int tile = blockIdx.x; tile * 32 < rows; tile += gridDim.x) {
+        const int row = tile * 32 + (lane & 31);
+        const bool valid = row < rows;
+        bf16x4 xb[9], h1[8], h2[8], h3[8];
+        f32x16 a4;
+        load_x(X, row, valid, nm.mean, nm.inv_std, hi, xb);
+        mlp_fwd(fr, P, xb, lane, h1, h2, h3, a4);
+        if (valid && hi == 0) {
+            Y[(size_t)row * 3 + 0] = a4[0] * nm.out_scale;
+            Y[(size_t)row * 3 + 1] = a4[1] * nm.out_scale;
+            Y[(size_t)row * 3 + 2] = a4[2] * nm.out_scale;
+        }
+    }
+}
+
+// ---- training ----
+
+// LDS transposes of one tile: T[unit][node] holds an activation (or the
+// input), U[unit][node] an output gradient, both BF16.
+struct TileLds {
+    uint16_t T[96 * 32];
+    uint16_t U[64 * 32];
+};
+
+__device__ __forceinline__ void store_act(uint16_t* T, const bf16x4* h, int nkb, int lane) {
+    const int node = lane & 31, hi = lane >> 5;
+    for (int kb = 0; kb < nkb; ++kb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) T[(kb * 8 + 4 * hi + i) * 32 + node] = (uint16_t)h[kb][i];
+}
+
+__device__ __forceinline__ void store_grad(uint16_t* U, const f32x16* g, int nmb, int lane) {
+    const int node = lane & 31, hi = lane >> 5;
+    for (int mb = 0; mb < nmb; ++mb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) U[(mb * 32 + 8 * (r >> 2) + 4 * hi + (r & 3)) * 32 + node] = f2bf(g[mb][r]);
+}
+
+__device__ __forceinline__ bf16x4 read_frag(const uint16_t* S, int unit0, int kbn, int lane) {
+    return *reinterpret_cast<const bf16x4*>(S + (unit0 + (lane & 31)) * 32 + kbn * 8 + 4 * (lane >> 5));
+}
+
+// dW[in][out] += sum over the tile's nodes of act[node][in] grad[node][out]
+// for in < nin, out < nout (atomics into the FP32 gradient).
+__device__ __forceinline__ void accumulate_dw(const TileLds& s, int nmi, int nno, int nin, int nout,
+                                              float* __restrict__ gw, int lane) {
+    const int hi = lane >> 5;
+    for (int mi = 0; mi < nmi; ++mi)
+        for (int no = 0; no < nno; ++no) {
+            f32x16 acc = {};
+#pragma unroll
+            for (int kbn = 0; kbn < 4; ++kbn) acc = mfma(read_frag(s.T, mi * 32, kbn, lane), read_frag(s.U, no * 32, kbn, lane), acc);
+            const int out = no * 32 + (lane & 31);
+            if (out < nout) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int in = mi * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+                    if (in < nin) atomicAdd(gw + in * nout + out, acc[r]);
+                }
+            }
+        }
+}
+
+// db[unit] += sum over nodes of g (FP32, D layout).
+__device__ __forceinline__ void accumulate_db(const f32x16* g, int nmb, int nvalid, float* __restrict__ gb, int lane) {
+    const int hi = lane >> 5;
+    for (int mb = 0; mb < nmb; ++mb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            float v = g[mb][r];
+#pragma unroll
+            for (int o = 1; o < 32; o <<= 1) v += __shfl_xor(v, o);
+            const int u = mb * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+            if ((lane & 31) == 0 && u < nvalid) atomicAdd(gb + u, v);
+        }
+}
+
+// grad := W * g^T through the backward fragments, masked by ReLU' of the
+// layer input h (given as packed activations).
+__device__ __forceinline__ void backprop(const bf16x4* __restrict__ fr, int base, int nkb, const f32x16* g,
+                                         const bf16x4 h[8], int lane, f32x16 out[2]) {
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+        out[mb] = f32x16{};
+        for (int kb = 0; kb < nkb; ++kb) {
+            const int gm = kb >> 2, q = (kb & 3) * 4;
+            out[mb] = mfma(fr[(base + mb * nkb + kb) * 64 + lane], pack4(g[gm][q], g[gm][q + 1], g[gm][q + 2], g[gm][q + 3]),
+                           out[mb]);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int kb = mb * 4 + (r >> 2);
+            if (!(bf2f((uint16_t)h[kb][r & 3]) > 0.0f)) out[mb][r] = 0.0f;
+        }
+    }
+}
+
+// One optimisation step's gradient over rows [row0, row0 + n) of the dataset
+// (targets T in correction units): grad += dL/dP of
+// L = mean over rows and the 3 outputs of (MLP(x) - t / out_scale)^2; loss
+// sum accumulates into *loss.  One wave per 32-row tile.
+__global__ __launch_bounds__(64) void k_train(const float* __restrict__ X, const float* __restrict__ Tg, int row0,
+                                              int n, int nrows, Norm nm, const bf16x4* __restrict__ fr,
+                                              const float* __restrict__ P, float* __restrict__ grad,
+                                              float* __restrict__ loss) {
+    __shared__ TileLds s;
+    const int lane = threadIdx.x, hi = lane >> 5;
+    for (int i = lane; i < 24 * 32; i += 64) s.T[72 * 32 + i] = 0;  // input rows 72..95 stay zero
+    const float coef = 2.0f / (3.0f * (float)n);
+    const float inv_scale = 1.0f / nm.out_scale;
+    for (int tile = blockIdx.x; tile * 32 < n; tile += gridDim.x) {
+        const int local = tile * 32 + (lane & 31);
+        const bool valid = local < n;
+        const int row = (row0 + local) % nrows;
+        bf16x4 xb[9], h1[8], h2[8], h3[8];
+        f32x16 a4;
+        load_x(X, row, valid, nm.mean, nm.inv_std, hi, xb);
+        mlp_fwd(fr, P, xb, lane, h1, h2, h3, a4);
+        // output gradient (rows 0..2 live in lanes 0..31, registers 0..2)
+        f32x16 g4[2] = {};
+        float l2 = 0.0f;
+        if (valid && hi == 0) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const float d = a4[j] - Tg[(size_t)row * 3 + j] * inv_scale;
+                g4[0][j] = coef * d;
+                l2 += d * d;
+            }
+        }
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) l2 += __shfl_xor(l2, o);
+        if (lane == 0) atomicAdd(loss, l2);
+        // layer 4
+        __syncthreads();
+        store_act(s.T, h3, 8, lane);
+        store_grad(s.U, g4, 1, lane);
+        __syncthreads();
+        accumulate_dw(s, 2, 1, NH, NO, grad + P_W4, lane);
+        accumulate_db(g4, 1, NO, grad + P_B4, lane);
+        f32x16 g3[2];
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb) {
+            g3[mb] = mfma(fr[(B_L4 + mb) * 64 + lane], pack4(g4[0][0], g4[0][1], g4[0][2], g4[0][3]), f32x16{});
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                if (!(bf2f((uint16_t)h3[mb * 4 + (r >> 2)][r & 3]) > 0.0f)) g3[mb][r] = 0.0f;
+        }
+        // layer 3
+        __syncthreads();
+        store_act(s.T, h2, 8, lane);
+        store_grad(s.U, g3, 2, lane);
+        __syncthreads();
+        accumulate_dw(s, 2, 2, NH, NH, grad + P_W3, lane);
+        accumulate_db(g3, 2, NH, grad + P_B3, lane);
+        f32x16 g2[2];
+        backprop(fr, B_L3, 8, g3, h2, lane, g2);
+        // layer 2
+        __syncthreads();
+        store_act(s.T, h1, 8, lane);
+        store_grad(s.U, g2, 2, lane);
+        __syncthreads();
+        accumulate_dw(s, 2, 2, NH, NH, grad + P_W2, lane);
+        accumulate_db(g2, 2, NH, grad + P_B2, lane);
+        f32x16 g1[2];
+        backprop(fr, B_L2, 8, g2, h1, lane, g1);
+        // layer 1
+        __syncthreads();
+        store_act(s.T, xb, 9, lane);
+        store_grad(s.U, g1, 2, lane);
+        __syncthreads();
+        accumulate_dw(s, 3, 2, NF, NH, grad + P_W1, lane);
+        accumulate_db(g1, 2, NH, grad + P_B1, lane);
+    }
+}
+
+__global__ void k_adam(float* __restrict__ P, float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    P[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+    g[i] = 0.0f;
+}
+
+__global__ void k_zero_f32(float* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+// ---- grid side: active nodes, features, corrections ----
+
+// Appends every node with mass > 1e-9 of the flagged 4^3 blocks (all scenes)
+// to `list` (x | y << 10 | z << 20 and the scene).  One wave per block.
+__global__ __launch_bounds__(256) void k_active(const DevCfg* __restrict__ cfgs, const uint16_t* __restrict__ block_scene,
+                                                GridBuf V, int nblocks, int2* __restrict__ list, int cap,
+                                                uint32_t* __restrict__ count) {
+    const int blk = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (blk >= nblocks) return;
+    if (V.flags[blk] == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int sc = block_scene[blk];
+    const DevCfg c = cfgs[sc];
+    const int lb = blk - c.foff;
+    const int bx = lb % c.fbx, by = (lb / c.fbx) % c.fby, bz = lb / (c.fbx * c.fby);
+    const int x = bx * 4 + (lane & 3), y = by * 4 + ((lane >> 2) & 3), z = bz * 4 + (lane >> 4);
+    const bool act = inb(c, x, y, z) && V.plane[0][gidx(c, x, y, z)] > 1e-9f;
+    const uint64_t m = __ballot(act);
+    if (m == 0) return;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, 0);
+    const uint32_t slot = base + (uint32_t)__popcll(m & (lane ? (~0ull >> (64 - lane)) : 0ull));
+    if (act && slot < (uint32_t)cap) list[slot] = make_int2(x | (y << 10) | (z << 20), sc);
+}
+
+// Per-node features (66 used of a 72-float row), from the grid after the
+// grid update (mass plane, velocity planes) and the SDF:
+//   0      mass / 8
+//   1..3   velocity * DT (cells per substep)
+//   4      SDF clamped to [-4, 4] / 4
+//   5..7   SDF normal (getNormalAtNode)
+//   8..34  masses / 8 of the 3^3 neighbourhood (x fastest; 0 outside the grid)
+//   35..61 momentum magnitudes |m v| * DT / 8 of the neighbourhood
+//   62     GRAVITY * DT^2,  63  COMPRESSION * DT^2
+//   64     distance to the nearest padded wall / 8, clamped to [0, 1]
+//   65     y / SIZE_Y
+__device__ void node_features(const DevCfg& c, const GridBuf& V, const float* __restrict__ sdf, int x, int y, int z,
+                              float* __restrict__ f) {
+    const int g = gidx(c, x, y, z);
+    const float dt = c.dt;
+    f[0] = V.plane[0][g] * 0.125f;
+    f[1] = V.plane[1][g] * dt;
+    f[2] = V.plane[2][g] * dt;
+    f[3] = V.plane[3][g] * dt;
+    f[4] = fminf(fmaxf(sdf[g], -4.0f), 4.0f) * 0.25f;
+    float nx, ny, nz;
+    node_normal(c, sdf, x, y, z, nx, ny, nz);
+    f[5] = nx; f[6] = ny; f[7] = nz;
+    int o = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx, ++o) {
+                float mm = 0.0f, pm = 0.0f;
+                if (inb(c, x + dx, y + dy, z + dz)) {
+                    const int q = gidx(c, x + dx, y + dy, z + dz);
+                    mm = V.plane[0][q];
+                    const float vx = V.plane[1][q], vy = V.plane[2][q], vz = V.plane[3][q];
+                    pm = mm * sqrtf(vx * vx + vy * vy + vz * vz) * dt;
+                }
+                f[8 + o] = mm * 0.125f;
+                f[35 + o] = pm * 0.125f;
+            }
+    f[62] = c.gravity * dt * dt;
+    f[63] = c.compression * dt * dt;
+    const float p = (float)c.pad;
+    float w = fminf(fminf((float)x - p, (float)(c.sx - 1) - p - (float)x),
+                    fminf(fminf((float)y - p, (float)(c.sy - 1) - p - (float)y),
+                          fminf((float)z - p, (float)(c.sz - 1) - p - (float)z)));
+    f[64] = fminf(fmaxf(w * 0.125f, 0.0f), 1.0f);
+    f[65] = (float)y / (float)c.sy;
+    for (int i = NF; i < KIN; ++i) f[i] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_features(const DevCfg* __restrict__ cfgs, GridBuf V,
+                                                  const float* __restrict__ sdf, const int2* __restrict__ list,
+                                                  const uint32_t* __restrict__ count, int cap, float* __restrict__ X) {
+    const int n = (int)min(*count, (uint32_t)cap);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int2 e = list[i];
+        const DevCfg c = cfgs[e.y];
+        node_features(c, V, sdf, e.x & 1023, (e.x >> 10) & 1023, e.x >> 20, X + (size_t)i * KIN);
+    }
+}
+
+// V[list node] += Y[row] (the corrections).
+__global__ void k_apply(const DevCfg* __restrict__ cfgs, GridBuf V, const int2* __restrict__ list,
+                        const uint32_t* __restrict__ count, int cap, const float* __restrict__ Y) {
+    const int n = (int)min(*count, (uint32_t)cap);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int2 e = list[i];
+        const DevCfg& c = cfgs[e.y];
+        const int g = gidx(c, e.x & 1023, (e.x >> 10) & 1023, e.x >> 20);
+        V.plane[1][g] += Y[(size_t)i * 3 + 0];
+        V.plane[2][g] += Y[(size_t)i * 3 + 1];
+        V.plane[3][g] += Y[(size_t)i * 3 + 2];
+    }
+}
+
+// Training pairs: the velocities of the listed nodes (after a grid update),
+// copied (mode 0) or accumulated with a count of the substeps in which the
+// node had mass (mode 1).
+__global__ void k_gather_vel(const DevCfg* __restrict__ cfgs, GridBuf V, const int2* __restrict__ list,
+                             const uint32_t* __restrict__ count, int cap, float* __restrict__ out,
+                             float* __restrict__ hits, int mode) {
+    const int n = (int)min(*count, (uint32_t)cap);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int2 e = list[i];
+        const DevCfg& c = cfgs[e.y];
+        const int g = gidx(c, e.x & 1023, (e.x >> 10) & 1023, e.x >> 20);
+        const bool live = V.plane[0][g] > 1e-9f;
+        const float vx = live ? V.plane[1][g] : 0.0f, vy = live ? V.plane[2][g] : 0.0f, vz = live ? V.plane[3][g] : 0.0f;
+        if (mode == 0) {
+            out[(size_t)i * 3] = vx; out[(size_t)i * 3 + 1] = vy; out[(size_t)i * 3 + 2] = vz;
+        } else {
+            out[(size_t)i * 3] += vx; out[(size_t)i * 3 + 1] += vy; out[(size_t)i * 3 + 2] += vz;
+            hits[i] += live ? 1.0f : 0.0f;
+        }
+    }
+}
+
+// Fallback checks over the particles: [0] particles whose position samples
+// SDF < -depth (collider penetration), [1] sum of |det F - 1|, [2] particles
+// with a non-finite position or velocity.
+__global__ __launch_bounds__(256) void k_check(const float* __restrict__ part, size_t stride, const int* __restrict__ orig,
+                                               int n, SceneTable t, const float* __restrict__ sdf, float depth,
+                                               float* __restrict__ out) {
+    float pen = 0.0f, drift = 0.0f, bad = 0.0f;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        const DevCfg& c = t.cfgs[scene_of(t, orig[s])];
+        const float x = part[A_PX * stride + s], y = part[A_PY * stride + s], z = part[A_PZ * stride + s];
+        const float vx = part[A_VX * stride + s], vy = part[A_VY * stride + s], vz = part[A_VZ * stride + s];
+        if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(vx) && isfinite(vy) && isfinite(vz))) {
+            bad += 1.0f;
+            continue;
+        }
+        if (sdf_at(c, sdf, x, y, z) < -depth) pen += 1.0f;
+        float F[9];
+        for (int i = 0; i < 9; ++i) F[i] = part[(A_F0 + i) * stride + s];
+        drift += fabsf(det3(F) - 1.0f);
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        pen += __shfl_xor(pen, o);
+        drift += __shfl_xor(drift, o);
+        bad += __shfl_xor(bad, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(out, pen);
+        atomicAdd(out + 1, drift);
+        atomicAdd(out + 2, bad);
+    }
+}
+
+int grid_blocks(int n, int per, int cap) { return std::max(1, std::min((n + per - 1) / per, cap)); }
+
+}  // namespace
+
+void corr_pack(const float* P, void* frags, hipStream_t s) {
+    hipLaunchKernelGGL(k_pack, dim3((NFRAG * 64 + 255) / 256), dim3(256), 0, s, P, (bf16x4*)frags);
+}
+
+void launch_corr_active(const DevCfg* cfgs, const uint16_t* block_scene, GridBuf V, int nblocks, int2* list, int cap,
+                        uint32_t* count, hipStream_t s) {
+    if (nblocks <= 0) return;
+    hipLaunchKernelGGL(k_active, dim3((nblocks + 3) / 4), dim3(256), 0, s, cfgs, block_scene, V, nblocks, list, cap,
+                       count);
+}
+
+void launch_corr_features(const DevCfg* cfgs, GridBuf V, const float* sdf, const int2* list, const uint32_t* count,
+                          int cap, float* X, hipStream_t s) {
+    hipLaunchKernelGGL(k_features, dim3(grid_blocks(cap, 256, 4096)), dim3(256), 0, s, cfgs, V, sdf, list, count, cap, X);
+}
+
+void launch_corr_forward(const CorrState& cs, const float* X, int cap, const uint32_t* count, float* Y, hipStream_t s) {
+    Norm nm{cs.mean, cs.inv_std, cs.out_scale};
+    hipLaunchKernelGGL(k_forward, dim3(grid_blocks(cap, 32, 8192)), dim3(64), 0, s, X, cap, count, nm,
+                       (const bf16x4*)cs.frags, cs.params, Y);
+}
+
+void launch_corr_apply(const DevCfg* cfgs, GridBuf V, const int2* list, const uint32_t* count, int cap, const float* Y,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_apply, dim3(grid_blocks(cap, 256, 4096)), dim3(256), 0, s, cfgs, V, list, count, cap, Y);
+}
+
+void launch_corr_gather_vel(const DevCfg* cfgs, GridBuf V, const int2* list, const uint32_t* count, int cap, float* out,
+                            float* hits, int mode, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_vel, dim3(grid_blocks(cap, 256, 4096)), dim3(256), 0, s, cfgs, V, list, count, cap,
+                       out, hits, mode);
+}
+
+void launch_check(const float* part, size_t stride, const int* orig, int n, const SceneTable& t, const float* sdf,
+                  float depth, float* out3, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_check, dim3(grid_blocks(n, 256, 2048)), dim3(256), 0, s, part, stride, orig, n, t, sdf, depth,
+                       out3);
+}
+
+}  // namespace mpm
+
+// ---------------- corrector object and its C ABI ----------------
+
+using namespace mpm;
+
+namespace {
+
+int cfail(int code, const std::string& msg) {
+    mpm::set_last_error(msg);
+    return code;
+}
+
+#define CHK(expr)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess) return cfail(MPM_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T> int alloc(T** p, size_t n) {
+    *p = nullptr;
+    if (hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return cfail(MPM_ENOMEM, "hipMalloc failed");
+    return MPM_OK;
+}
+
+}  // namespace
+
+struct mpm_corrector {
+    CorrState st{};
+    float* grad = nullptr;
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    int64_t adam_t = 0;
+    float* loss = nullptr;  // per training step
+    int loss_cap = 0;
+    float* dX = nullptr;  // dataset
+    float* dT = nullptr;
+    int nrows = 0;
+    hipStream_t stream = nullptr;
+};
+
+namespace mpm {
+const CorrState* corrector_state(const mpm_corrector* c) { return c ? &c->st : nullptr; }
+}  // namespace mpm
+
+extern "C" {
+
+int mpm_corrector_create(mpm_corrector** out) {
+    if (!out) return cfail(MPM_EINVAL, "null argument");
+    *out = nullptr;
+    mpm_corrector* c = new mpm_corrector();
+    int rc;
+    if ((rc = alloc(&c->st.params, CORR_PARAMS)) || (rc = alloc((uint16_t**)&c->st.frags, (size_t)NFRAG * 256)) ||
+        (rc = alloc(&c->st.mean, KIN)) || (rc = alloc(&c->st.inv_std, KIN)) || (rc = alloc(&c->grad, CORR_PARAMS)) ||
+        (rc = alloc(&c->adam_m, CORR_PARAMS)) || (rc = alloc(&c->adam_v, CORR_PARAMS))) {
+        mpm_corrector_destroy(c);
+        return rc;
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        mpm_corrector_destroy(c);
+        return cfail(MPM_EHIP, "hipStreamCreate failed");
+    }
+    std::vector<float> zero(CORR_PARAMS, 0.0f), mean(KIN, 0.0f), inv(KIN, 0.0f);
+    for (int i = 0; i < NF; ++i) inv[i] = 1.0f;
+    c->st.out_scale = 1.0f;
+    hipMemcpy(c->st.params, zero.data(), sizeof(float) * CORR_PARAMS, hipMemcpyHostToDevice);
+    hipMemcpy(c->grad, zero.data(), sizeof(float) * CORR_PARAMS, hipMemcpyHostToDevice);
+    hipMemcpy(c->adam_m, zero.data(), sizeof(float) * CORR_PARAMS, hipMemcpyHostToDevice);
+    hipMemcpy(c->adam_v, zero.data(), sizeof(float) * CORR_PARAMS, hipMemcpyHostToDevice);
+    hipMemcpy(c->st.mean, mean.data(), sizeof(float) * KIN, hipMemcpyHostToDevice);
+    hipMemcpy(c->st.inv_std, inv.data(), sizeof(float) * KIN, hipMemcpyHostToDevice);
+    corr_pack(c->st.params, c->st.frags, c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        mpm_corrector_destroy(c);
+        return cfail(MPM_EHIP, "corrector init failed");
+    }
+    *out = c;
+    return MPM_OK;
+}
+
+int mpm_corrector_destroy(mpm_corrector* c) {
+    if (!c) return MPM_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    hipFree(c->st.params); hipFree(c->st.frags); hipFree(c->st.mean); hipFree(c->st.inv_std);
+    hipFree(c->grad); hipFree(c->adam_m); hipFree(c->adam_v); hipFree(c->loss); hipFree(c->dX); hipFree(c->dT);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return MPM_OK;
+}
+
+int mpm_corrector_set_params(mpm_corrector* c, const float* params, int n) {
+    if (!c || !params || n != CORR_PARAMS) return cfail(MPM_EINVAL, "parameter count must be MPM_CORRECTOR_PARAMS");
+    CHK(hipMemcpyAsync(c->st.params, params, sizeof(float) * CORR_PARAMS, hipMemcpyHostToDevice, c->stream));
+    corr_pack(c->st.params, c->st.frags, c->stream);
+    std::vector<float> zero(CORR_PARAMS, 0.0f);
+    CHK(hipMemcpyAsync(c->adam_m, zero.data(), sizeof(float) * CORR_PARAMS, hipMemcpyHostToDevice, c->stream));
+    CHK(hipMemcpyAsync(c->adam_v, zero.data(), sizeof(float) * CORR_PARAMS, hipMemcpyHostToDevice, c->stream));
+    c->adam_t = 0;
+    CHK(hipStreamSynchronize(c->stream));
+    return MPM_OK;
+}
+
+int mpm_corrector_get_params(mpm_corrector* c, float* params, int n) {
+    if (!c || !params || n != CORR_PARAMS) return cfail(MPM_EINVAL, "parameter count must be MPM_CORRECTOR_PARAMS");
+    CHK(hipMemcpyAsync(params, c->st.params, sizeof(float) * CORR_PARAMS, hipMemcpyDeviceToHost, c->stream));
+    CHK(hipStreamSynchronize(c->stream));
+    return MPM_OK;
+}
+
+int mpm_corrector_set_norm(mpm_corrector* c, const float* mean, const float* inv_std, float out_scale) {
+    if (!c || !mean || !inv_std || !(out_scale > 0.0f)) return cfail(MPM_EINVAL, "bad normalisation");
+    std::vector<float> m(KIN, 0.0f), s(KIN, 0.0f);
+    for (int i = 0; i < NF; ++i) {
+        m[i] = mean[i];
+        s[i] = inv_std[i];
+    }
+    CHK(hipMemcpyAsync(c->st.mean, m.data(), sizeof(float) * KIN, hipMemcpyHostToDevice, c->stream));
+    CHK(hipMemcpyAsync(c->st.inv_std, s.data(), sizeof(float) * KIN, hipMemcpyHostToDevice, c->stream));
+    CHK(hipStreamSynchronize(c->stream));
+    c->st.out_scale = out_scale;
+    return MPM_OK;
+}
+
+int mpm_corrector_forward(mpm_corrector* c, const float* X, int n, float* Y) {
+    if (!c || n < 0 || (n > 0 && (!X || !Y))) return cfail(MPM_EINVAL, "bad arguments");
+    if (n == 0) return MPM_OK;
+    float *dx = nullptr, *dy = nullptr;
+    int rc;
+    if ((rc = alloc(&dx, (size_t)n * KIN)) || (rc = alloc(&dy, (size_t)n * 3))) {
+        hipFree(dx);
+        return rc;
+    }
+    hipMemcpyAsync(dx, X, sizeof(float) * KIN * (size_t)n, hipMemcpyHostToDevice, c->stream);
+    launch_corr_forward(c->st, dx, n, nullptr, dy, c->stream);
+    hipMemcpyAsync(Y, dy, sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToHost, c->stream);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    hipFree(dx);
+    hipFree(dy);
+    if (e != hipSuccess) return cfail(MPM_EHIP, std::string("corrector forward: ") + hipGetErrorString(e));
+    return MPM_OK;
+}
+
+int mpm_corrector_load_dataset(mpm_corrector* c, const float* X, const float* T, int n) {
+    if (!c || n < 1 || !X || !T) return cfail(MPM_EINVAL, "bad dataset");
+    hipFree(c->dX);
+    hipFree(c->dT);
+    c->dX = c->dT = nullptr;
+    c->nrows = 0;
+    int rc;
+    if ((rc = alloc(&c->dX, (size_t)n * KIN)) || (rc = alloc(&c->dT, (size_t)n * 3))) return rc;
+    CHK(hipMemcpyAsync(c->dX, X, sizeof(float) * KIN * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    CHK(hipMemcpyAsync(c->dT, T, sizeof(float) * 3 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    CHK(hipStreamSynchronize(c->stream));
+    c->nrows = n;
+    return MPM_OK;
+}
+
+// Gradient of the loss over rows [row0, row0 + n) of the loaded dataset
+// (wrapping), without an update: grad_out (CORR_PARAMS) and the loss.
+int mpm_corrector_gradient(mpm_corrector* c, int row0, int n, float* grad_out, float* loss_out) {
+    if (!c || n < 1 || !grad_out || !loss_out) return cfail(MPM_EINVAL, "bad arguments");
+    if (!c->nrows) return cfail(MPM_ESTATE, "no dataset loaded");
+    if (!c->loss) {
+        int rc;
+        if ((rc = alloc(&c->loss, 1))) return rc;
+        c->loss_cap = 1;
+    }
+    Norm nm{c->st.mean, c->st.inv_std, c->st.out_scale};
+    hipLaunchKernelGGL(k_zero_f32, dim3((CORR_PARAMS + 255) / 256), dim3(256), 0, c->stream, c->grad, CORR_PARAMS);
+    hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, c->stream, c->loss, 1);
+    hipLaunchKernelGGL(k_train, dim3(grid_blocks(n, 32, 1024)), dim3(64), 0, c->stream, c->dX, c->dT, row0, n,
+                       c->nrows, nm, (const bf16x4*)c->st.frags, c->st.params, c->grad, c->loss);
+    CHK(hipMemcpyAsync(grad_out, c->grad, sizeof(float) * CORR_PARAMS, hipMemcpyDeviceToHost, c->stream));
+    float l = 0.0f;
+    CHK(hipMemcpyAsync(&l, c->loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    CHK(hipStreamSynchronize(c->stream));
+    hipLaunchKernelGGL(k_zero_f32, dim3((CORR_PARAMS + 255) / 256), dim3(256), 0, c->stream, c->grad, CORR_PARAMS);
+    CHK(hipStreamSynchronize(c->stream));
+    *loss_out = l / (3.0f * (float)n);
+    return MPM_OK;
+}
+
+// `steps` Adam steps over consecutive batches of the loaded dataset (row
+// offset advancing by `batch` each step, wrapping); loss_out[s] = the mean
+// loss of step s's batch before its update.
+int mpm_corrector_train(mpm_corrector* c, int steps, int batch, float lr, float* loss_out) {
+    if (!c || steps < 1 || batch < 32 || !(lr > 0.0f)) return cfail(MPM_EINVAL, "bad training arguments");
+    if (!c->nrows) return cfail(MPM_ESTATE, "no dataset loaded");
+    if (c->loss_cap < steps) {
+        hipFree(c->loss);
+        c->loss = nullptr;
+        int rc;
+        if ((rc = alloc(&c->loss, (size_t)steps))) return rc;
+        c->loss_cap = steps;
+    }
+    hipLaunchKernelGGL(k_zero_f32, dim3((steps + 255) / 256), dim3(256), 0, c->stream, c->loss, steps);
+    Norm nm{c->st.mean, c->st.inv_std, c->st.out_scale};
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int s = 0; s < steps; ++s) {
+        const int row0 = (int)(((int64_t)s * batch + c->adam_t * 7919) % c->nrows);
+        hipLaunchKernelGGL(k_train, dim3(grid_blocks(batch, 32, 1024)), dim3(64), 0, c->stream, c->dX, c->dT, row0,
+                           batch, c->nrows, nm, (const bf16x4*)c->st.frags, c->st.params, c->grad, c->loss + s);
+        ++c->adam_t;
+        const float c1 = 1.0f / (1.0f - std::pow(b1, (float)c->adam_t));
+        const float c2 = 1.0f / (1.0f - std::pow(b2, (float)c->adam_t));
+        hipLaunchKernelGGL(k_adam, dim3((CORR_PARAMS + 255) / 256), dim3(256), 0, c->stream, c->st.params, c->grad,
+                           c->adam_m, c->adam_v, CORR_PARAMS, lr, b1, b2, eps, c1, c2);
+        corr_pack(c->st.params, c->st.frags, c->stream);
+    }
+    if (loss_out) {
+        std::vector<float> l((size_t)steps);
+        CHK(hipMemcpyAsync(l.data(), c->loss, sizeof(float) * steps, hipMemcpyDeviceToHost, c->stream));
+        CHK(hipStreamSynchronize(c->stream));
+        for (int s = 0; s < steps; ++s) loss_out[s] = l[s] / (3.0f * (float)batch);
+    }
+    CHK(hipStreamSynchronize(c->stream));
+    return MPM_OK;
+}
+
+}  // extern "C"

@@ -41,7 +41,7 @@ int fail(int code, const std::string& msg) {
         if (e_ != hipSuccess) return fail(MPM_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-enum Stage { ST_P2G = 0, ST_G2P = 1, ST_G2P2G = 2, ST_GRID = 3, ST_SORT = 4, ST_COUNT = 5 };
+enum Stage { ST_P2G = 0, ST_G2P = 1, ST_G2P2G = 2, ST_GRID = 3, ST_SORT = 4, ST_CORR = 5, ST_COUNT = 6 };
 
 constexpr int MAX_SCENES = 4096;
 
@@ -118,6 +118,19 @@ struct mpm_engine {
         int end_vcur = 0, end_pcur = 0;
     } graph;
     int64_t graph_replays = 0;
+
+    // Learned corrector: applied after every grid update while `correct`.
+    const CorrState* corr = nullptr;
+    bool correct = false;
+    bool pair_mode = false;  // collect_pairs: accumulate fine-step velocities after each grid update
+    int2* clist = nullptr;   // active nodes
+    float* cX = nullptr;     // features, CORR_STRIDE per node
+    float* cY = nullptr;     // corrections / coarse velocities, 3 per node
+    float* cacc = nullptr;   // accumulated fine velocities, 3 per node
+    float* chits = nullptr;
+    int ccap = 0;
+    float dt_scale = 1.0f;
+    float* check_buf = nullptr;
 
     // Saved particle state (mpm_engine_snapshot / mpm_engine_restore).
     float* snap_planes = nullptr;  // A_STATE x n
@@ -344,11 +357,26 @@ void run_transfer(mpm_engine* h, int mode, int launch) {
     launch_transfer(mode, make_args(h), launch, h->stream);
 }
 
+uint32_t* corr_count(mpm_engine* h) { return h->sc.counters + 12; }
+
 void run_grid_update(mpm_engine* h) {
-    StageTimer t(h, ST_GRID);
-    launch_grid_update(h->d_cfgs, h->d_block_scene, h->grid[1 - h->vcur], h->grid[h->vcur], true, h->sdf, h->nfb,
-                       h->stream);
-    h->vcur = 1 - h->vcur;
+    {
+        StageTimer t(h, ST_GRID);
+        launch_grid_update(h->d_cfgs, h->d_block_scene, h->grid[1 - h->vcur], h->grid[h->vcur], true, h->sdf, h->nfb,
+                           h->stream);
+        h->vcur = 1 - h->vcur;
+    }
+    const GridBuf& V = h->grid[h->vcur];
+    if (h->correct && h->corr && h->ccap > 0) {
+        StageTimer t(h, ST_CORR);
+        launch_set_u32(corr_count(h), 0, 0u, h->stream);
+        launch_corr_active(h->d_cfgs, h->d_block_scene, V, h->nfb, h->clist, h->ccap, corr_count(h), h->stream);
+        launch_corr_features(h->d_cfgs, V, h->sdf, h->clist, corr_count(h), h->ccap, h->cX, h->stream);
+        launch_corr_forward(*h->corr, h->cX, h->ccap, corr_count(h), h->cY, h->stream);
+        launch_corr_apply(h->d_cfgs, V, h->clist, corr_count(h), h->ccap, h->cY, h->stream);
+    }
+    if (h->pair_mode)
+        launch_corr_gather_vel(h->d_cfgs, V, h->clist, corr_count(h), h->ccap, h->cacc, h->chits, 1, h->stream);
 }
 
 int zero_grid(mpm_engine* h, int which) {
@@ -622,6 +650,24 @@ int sdf_from_triangles(mpm_engine* h, int scene, const mpm_triangle* tris, int n
     return MPM_OK;
 }
 
+int ensure_corr_buffers(mpm_engine* h) {
+    if (h->ccap > 0) return MPM_OK;
+    const size_t cap = h->G;
+    int rc;
+    if ((rc = dev_alloc(&h->clist, cap)) || (rc = dev_alloc(&h->cX, cap * CORR_STRIDE)) ||
+        (rc = dev_alloc(&h->cY, cap * 3)) || (rc = dev_alloc(&h->cacc, cap * 3)) || (rc = dev_alloc(&h->chits, cap)))
+        return rc;
+    h->ccap = (int)cap;
+    return MPM_OK;
+}
+
+int upload_dt(mpm_engine* h, float scale) {
+    for (int s = 0; s < h->S; ++s) h->dcs[s].dt = h->cfgs[s].DT * scale;
+    HIPCHK(hipMemcpyAsync(h->d_cfgs, h->dcs.data(), (size_t)h->S * sizeof(DevCfg), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return MPM_OK;
+}
+
 int copy_grid_out(mpm_engine* h, int which, size_t off, size_t count, float* out, size_t out_stride) {
     for (int p = 0; p < 4; ++p)
         HIPCHK(hipMemcpyAsync(out + p * out_stride, h->grid[which].plane[p] + off, count * sizeof(float),
@@ -663,6 +709,12 @@ int mpm_engine_destroy(mpm_engine* h) {
         dev_free(h->grid[b].flags);
     }
     dev_free(h->sdf);
+    dev_free(h->clist);
+    dev_free(h->cX);
+    dev_free(h->cY);
+    dev_free(h->cacc);
+    dev_free(h->chits);
+    dev_free(h->check_buf);
     dev_free(h->d_cfgs);
     dev_free(h->d_block_scene);
     dev_free(h->sc.counters);
@@ -709,6 +761,10 @@ int mpm_engine_set_option(mpm_engine* h, const char* key, int64_t value) {
     } else if (k == "GRAPH") {
         h->use_graph = value != 0;
         if (!h->use_graph) drop_graph(h);
+    } else if (k == "CORRECT") {
+        if (value && !h->corr) return fail(MPM_ESTATE, "no corrector attached (mpm_engine_set_corrector)");
+        drop_graph(h);
+        h->correct = value != 0;
     } else {
         return fail(MPM_EINVAL, "unknown option " + k);
     }
@@ -723,6 +779,7 @@ int mpm_engine_get_option(mpm_engine* h, const char* key, int64_t* value) {
     else if (k == "LAUNCH_PCT") *value = h->launch_pct;
     else if (k == "TIMING") *value = h->timing ? 1 : 0;
     else if (k == "GRAPH") *value = h->use_graph ? 1 : 0;
+    else if (k == "CORRECT") *value = h->correct ? 1 : 0;
     else if (k == "GRAPH_REPLAYS") *value = h->graph_replays;
     else if (k == "SCENES") *value = h->S;
     else return fail(MPM_EINVAL, "unknown option " + k);
@@ -813,6 +870,126 @@ int mpm_engine_steps(mpm_engine* h, int k) {
 }
 
 int mpm_engine_step(mpm_engine* h) { return mpm_engine_steps(h, 1); }
+
+int mpm_engine_set_dt_scale(mpm_engine* h, float scale) {
+    int rc = check_engine(h, false);
+    if (rc) return rc;
+    if (!(scale > 0.0f) || scale > 1e6f) return fail(MPM_EINVAL, "DT scale must be positive");
+    if ((rc = upload_dt(h, scale))) return rc;
+    h->dt_scale = scale;
+    return MPM_OK;
+}
+
+int mpm_engine_set_corrector(mpm_engine* h, mpm_corrector* c) {
+    int rc = check_engine(h, false);
+    if (rc) return rc;
+    if (c && (rc = ensure_corr_buffers(h))) return rc;
+    drop_graph(h);
+    h->corr = corrector_state(c);
+    if (!c) h->correct = false;
+    return MPM_OK;
+}
+
+int mpm_engine_check(mpm_engine* h, float depth, double* out3) {
+    int rc = check_engine(h, true);
+    if (rc) return rc;
+    if (!out3) return fail(MPM_EINVAL, "null output");
+    if (!h->check_buf && (rc = dev_alloc(&h->check_buf, 4))) return rc;
+    HIPCHK(hipMemsetAsync(h->check_buf, 0, 4 * sizeof(float), h->stream));
+    launch_check(h->planes[h->pcur], h->plane_cap[h->pcur], h->orig[h->pcur], h->n, scene_table(h), h->sdf, depth,
+                 h->check_buf, h->stream);
+    float v[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(v, h->check_buf, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    out3[0] = v[0];
+    out3[1] = h->n ? v[1] / h->n : 0.0;
+    out3[2] = v[2];
+    return MPM_OK;
+}
+
+int mpm_engine_debug_features(mpm_engine* h, int32_t* nodes_out, float* X_out, int cap, int* count_out) {
+    int rc = check_engine(h, false);
+    if (rc) return rc;
+    if (cap < 0 || !count_out || (cap > 0 && (!nodes_out || !X_out))) return fail(MPM_EINVAL, "bad arguments");
+    if ((rc = ensure_corr_buffers(h))) return rc;
+    const GridBuf& V = h->grid[h->vcur];
+    launch_set_u32(corr_count(h), 0, 0u, h->stream);
+    launch_corr_active(h->d_cfgs, h->d_block_scene, V, h->nfb, h->clist, h->ccap, corr_count(h), h->stream);
+    launch_corr_features(h->d_cfgs, V, h->sdf, h->clist, corr_count(h), h->ccap, h->cX, h->stream);
+    if ((rc = read_counters(h))) return rc;
+    const int rows = (int)std::min<uint32_t>(std::min<uint32_t>(h->pinned[12], (uint32_t)h->ccap), (uint32_t)cap);
+    if (rows > 0) {
+        std::vector<int2> l((size_t)rows);
+        HIPCHK(hipMemcpyAsync(l.data(), h->clist, l.size() * sizeof(int2), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(X_out, h->cX, (size_t)rows * CORR_STRIDE * sizeof(float), hipMemcpyDeviceToHost,
+                              h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int i = 0; i < rows; ++i) {
+            nodes_out[2 * i] = l[i].x;
+            nodes_out[2 * i + 1] = l[i].y;
+        }
+    }
+    *count_out = rows;
+    return MPM_OK;
+}
+
+int mpm_engine_collect_pairs(mpm_engine* h, int ratio, float* X_out, float* T_out, int cap, int* count_out) {
+    int rc = check_engine(h, true);
+    if (rc) return rc;
+    if (ratio < 2 || cap < 0 || !count_out || (cap > 0 && (!X_out || !T_out)))
+        return fail(MPM_EINVAL, "bad collect_pairs arguments");
+    *count_out = 0;
+    if (h->n == 0) return MPM_OK;
+    if ((rc = ensure_corr_buffers(h))) return rc;
+    drop_graph(h);
+    const bool was_correct = h->correct;
+    const float scale0 = h->dt_scale;
+    h->correct = false;
+    if ((rc = mpm_engine_snapshot(h))) return rc;
+    // coarse substep (DT * ratio): P2G + grid update, then the features and
+    // velocities of its active nodes
+    if ((rc = upload_dt(h, scale0 * ratio))) return rc;
+    if ((rc = resort(h, true))) return rc;
+    h->sorted_at = h->substeps;
+    run_transfer(h, XFER_P2G, transfer_launch_size(h));
+    run_grid_update(h);
+    const GridBuf& V = h->grid[h->vcur];
+    launch_set_u32(corr_count(h), 0, 0u, h->stream);
+    launch_corr_active(h->d_cfgs, h->d_block_scene, V, h->nfb, h->clist, h->ccap, corr_count(h), h->stream);
+    launch_corr_features(h->d_cfgs, V, h->sdf, h->clist, corr_count(h), h->ccap, h->cX, h->stream);
+    launch_corr_gather_vel(h->d_cfgs, V, h->clist, corr_count(h), h->ccap, h->cY, nullptr, 0, h->stream);
+    if ((rc = upload_dt(h, scale0))) return rc;
+    // back to the saved particles; `ratio` plain substeps accumulate the grid
+    // velocities of the same nodes
+    if ((rc = mpm_engine_restore(h))) return rc;
+    launch_fill_f32(h->cacc, (size_t)h->ccap * 3, 0.0f, h->stream);
+    launch_fill_f32(h->chits, (size_t)h->ccap, 0.0f, h->stream);
+    h->pair_mode = true;
+    rc = enqueue_steps(h, ratio, transfer_launch_size(h));
+    h->pair_mode = false;
+    h->correct = was_correct;
+    if (rc || (rc = finish_steps(h))) return rc;
+    const int rows = (int)std::min<uint32_t>(h->pinned[12], (uint32_t)h->ccap);
+    std::vector<float> X((size_t)rows * CORR_STRIDE), vc((size_t)rows * 3), acc((size_t)rows * 3), hits((size_t)rows);
+    if (rows > 0) {
+        HIPCHK(hipMemcpyAsync(X.data(), h->cX, X.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(vc.data(), h->cY, vc.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(acc.data(), h->cacc, acc.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(hits.data(), h->chits, hits.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    // keep nodes that had mass in every fine substep; target = mean fine
+    // velocity - coarse velocity
+    int w = 0;
+    for (int i = 0; i < rows && w < cap; ++i) {
+        if (hits[i] < (float)ratio) continue;
+        std::memcpy(X_out + (size_t)w * CORR_STRIDE, X.data() + (size_t)i * CORR_STRIDE, CORR_STRIDE * sizeof(float));
+        for (int j = 0; j < 3; ++j) T_out[(size_t)w * 3 + j] = acc[(size_t)i * 3 + j] / (float)ratio - vc[(size_t)i * 3 + j];
+        ++w;
+    }
+    *count_out = w;
+    return MPM_OK;
+}
 
 int mpm_engine_snapshot(mpm_engine* h) {
     int rc = check_engine(h, true);

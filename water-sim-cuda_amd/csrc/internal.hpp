@@ -6,9 +6,13 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "device_common.hpp"
 
 namespace mpm {
+
+void set_last_error(const std::string& msg);
 
 // Grid buffer: four planes of G floats (mass, x, y, z).  While it accumulates
 // P2G the planes hold mass and momentum; after the grid update the x/y/z
@@ -112,4 +116,40 @@ void launch_reorder(ChunkDesc* ch, const uint32_t* nchunks_dev, int chunk_est, c
                     size_t dst_stride, int* dst_orig, uint8_t* rounds, uint32_t* rounds_total, uint32_t attr_mask,
                     hipStream_t s);
 
+// ---- learned corrector (corrector.hip) ----
+constexpr int CORR_FEATURES = 66;
+constexpr int CORR_STRIDE = 72;     // floats per feature row (zero padded)
+constexpr int CORR_PARAMS = 12803;  // 66*64 + 64 + 2*(64*64 + 64) + 64*3 + 3
+
+struct CorrState {
+    float* params;    // FP32 master weights
+    void* frags;      // BF16 MFMA operand fragments derived from params
+    float* mean;      // CORR_STRIDE
+    float* inv_std;   // CORR_STRIDE
+    float out_scale;  // correction = output * out_scale
+};
+
+void corr_pack(const float* params, void* frags, hipStream_t s);
+// Appends the nodes with mass > 1e-9 of V's flagged blocks to list (count on
+// the device, at most cap entries stored).
+void launch_corr_active(const DevCfg* cfgs, const uint16_t* block_scene, GridBuf V, int nblocks, int2* list, int cap,
+                        uint32_t* count, hipStream_t s);
+void launch_corr_features(const DevCfg* cfgs, GridBuf V, const float* sdf, const int2* list, const uint32_t* count,
+                          int cap, float* X, hipStream_t s);
+void launch_corr_forward(const CorrState& cs, const float* X, int cap, const uint32_t* count, float* Y, hipStream_t s);
+void launch_corr_apply(const DevCfg* cfgs, GridBuf V, const int2* list, const uint32_t* count, int cap, const float* Y,
+                       hipStream_t s);
+// Velocities of the listed nodes: copied (mode 0) or accumulated with a count
+// of the calls in which the node had mass (mode 1).
+void launch_corr_gather_vel(const DevCfg* cfgs, GridBuf V, const int2* list, const uint32_t* count, int cap, float* out,
+                            float* hits, int mode, hipStream_t s);
+// out3 += {particles below SDF -depth, sum |det F - 1|, non-finite particles}.
+void launch_check(const float* part, size_t stride, const int* orig, int n, const SceneTable& t, const float* sdf,
+                  float depth, float* out3, hipStream_t s);
+
+}  // namespace mpm
+
+struct mpm_corrector;
+namespace mpm {
+const CorrState* corrector_state(const mpm_corrector* c);
 }  // namespace mpm

@@ -1,0 +1,233 @@
+"""Learned grid-velocity corrector: data, training, fallback runner and
+evaluation (no reference counterpart).
+
+The plain MLS-MPM substep at DT is the ground truth.  A corrected substep at
+RATIO*DT runs P2G and the grid update at the large step, adds the network's
+per-node velocity correction, and runs G2P.  Training pairs come from plain
+runs: at a state S, the features of the active nodes after one RATIO*DT grid
+update, and as targets the mean grid velocity of the same nodes over RATIO
+plain substeps from S minus the coarse velocity (Engine.collect_pairs).
+
+The plain step is the fallback: each frame of corrected substeps starts from
+a device snapshot, and when a check trips afterwards -- particles deeper than
+PEN_DEPTH inside the collider, mean |det F - 1| above DRIFT_MAX, or a
+non-finite particle -- the frame is rewound and rerun with RATIO times as many
+plain substeps.
+"""
+from __future__ import annotations
+
+import json
+import time
+from pathlib import Path
+
+import numpy as np
+
+RATIO = 4
+PEN_DEPTH = 0.5      # cells
+PEN_FRACTION = 1e-3  # of the particles
+DRIFT_MAX = 0.05     # mean |det F - 1|
+
+
+def _pkg():
+    import importlib
+
+    return importlib.import_module("water-sim-cuda_amd")
+
+
+def dambreak_family(seed: int, physical: bool = True) -> dict:
+    """Member `seed` of the 128^3 dam-break family: the 48x96x48 block of the
+    base scene shifted along z and resized by up to 8 cells, jitter seeded,
+    UV-sphere collider (2,048 triangles) moved within the basin."""
+    m = _pkg()
+    base = m.scenes.dambreak128(physical=physical, seed=seed)
+    rng = np.random.default_rng(1000 + seed)
+    dz = int(rng.integers(-8, 9))
+    w = int(rng.integers(40, 49))
+    h = int(rng.integers(88, 97))
+    lo = np.array([4.0, 4.0, 40.0 + dz], dtype=np.float32)
+    hi = lo + np.array([w, h, 48.0], dtype=np.float32)
+    parts = m.scene_fill("box", lo, hi, seed)
+    c = (88.3 + float(rng.uniform(-6, 6)), 24.3, 64.3 + float(rng.uniform(-10, 10)))
+    tris = m.mesh_uv_sphere(c, 20.0, 32, 33)
+    return dict(base, particles=parts, triangles=tris, name=f"dambreak128-s{seed}",
+                description=f"128^3 dam break {w}x{h}x48 at z={40 + dz}, sphere at {c[0]:.1f},{c[2]:.1f}")
+
+
+def make_engine(scene: dict, **opts):
+    m = _pkg()
+    e = m.Engine(scene["cfg"], **opts)
+    e.init(scene["particles"].shape[1], scene["particles"])
+    if scene.get("sdf") is not None:
+        e.set_sdf(scene["sdf"])
+    else:
+        e.initSDF(scene["triangles"])
+    return e
+
+
+def generate_pairs(seeds, samples: int, every: int = 100, per_sample: int = 16384, substeps: int = 2000,
+                   ratio: int = RATIO, rng_seed: int = 0, log=None):
+    """Training pairs from plain runs of the dam-break family: for each seed,
+    a plain run of `substeps` substeps at the scene's DT, collecting pairs
+    every `every` substeps, per_sample rows per collection (random subset),
+    until `samples` rows in all."""
+    rng = np.random.default_rng(rng_seed)
+    Xs, Ts = [], []
+    total = 0
+    quota = int(np.ceil(samples / max(1, len(seeds))))
+    for s in seeds:
+        sc = dambreak_family(s)
+        e = make_engine(sc, RESORT_EVERY=sc["resort_every"])
+        got = 0
+        t = 0
+        while t + ratio <= substeps and got < quota:
+            X, T = e.collect_pairs(ratio, 400000)
+            t += ratio
+            if len(X):
+                k = min(per_sample, len(X), quota - got)
+                idx = rng.choice(len(X), size=k, replace=False)
+                Xs.append(X[idx])
+                Ts.append(T[idx])
+                got += k
+            adv = every - ratio
+            if adv > 0 and t + adv <= substeps:
+                e.steps(adv)
+                t += adv
+        e.close()
+        total += got
+        if log:
+            log(f"pairs: seed {s} -> {got} rows (total {total})")
+        if total >= samples:
+            break
+    X = np.concatenate(Xs)[:samples]
+    T = np.concatenate(Ts)[:samples]
+    return X, T
+
+
+def normalisation(X: np.ndarray, T: np.ndarray):
+    """Per-feature mean and 1/std (1 for constant features), output scale =
+    RMS of the targets."""
+    mean = X[:, :66].mean(0).astype(np.float32)
+    std = X[:, :66].std(0)
+    inv = np.where(std > 1e-8, 1.0 / np.maximum(std, 1e-8), 0.0).astype(np.float32)
+    scale = float(np.sqrt((T.astype(np.float64) ** 2).mean())) or 1.0
+    return mean, inv, scale
+
+
+def train(corr, X, T, epochs: int = 20, batch: int = 4096, lr: float = 1e-3, seed: int = 0, log=None):
+    """Adam on the device; returns the per-step losses."""
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(len(X))
+    corr.load_dataset(X[perm], T[perm])
+    steps_per_epoch = max(1, len(X) // batch)
+    losses = []
+    for ep in range(epochs):
+        lr_ep = lr * (0.5 * (1 + np.cos(np.pi * ep / epochs)))  # cosine schedule
+        l = corr.train(steps_per_epoch, batch, max(lr_ep, lr * 0.02))
+        losses.append(l)
+        if log:
+            log(f"train: epoch {ep} loss {float(l.mean()):.4f}")
+    return np.concatenate(losses)
+
+
+class LearnedRunner:
+    """Steps an engine in frames of corrected RATIO*DT substeps with the plain
+    step as fallback (see module docstring)."""
+
+    def __init__(self, eng, corr, ratio: int = RATIO):
+        self.eng, self.corr, self.ratio = eng, corr, ratio
+        self.frames = 0
+        self.fallbacks = 0
+        eng.set_corrector(corr)
+
+    def frame(self, coarse_substeps: int) -> bool:
+        """Advances by coarse_substeps * ratio plain-DT worth of time; returns
+        False when the frame fell back to the plain step."""
+        e = self.eng
+        e.snapshot()
+        e.set_dt_scale(self.ratio)
+        e.set_option("CORRECT", 1)
+        e.steps(coarse_substeps)
+        e.set_option("CORRECT", 0)
+        e.set_dt_scale(1.0)
+        chk = e.check(PEN_DEPTH)
+        n = e.getNumber()
+        ok = (chk["nonfinite"] == 0 and chk["penetrating"] <= PEN_FRACTION * n
+              and chk["mean_abs_J_minus_1"] <= DRIFT_MAX)
+        self.frames += 1
+        if not ok:
+            self.fallbacks += 1
+            e.restore()
+            e.steps(coarse_substeps * self.ratio)
+        return ok
+
+
+def rms(a: np.ndarray, b: np.ndarray) -> float:
+    d = a[0:3].astype(np.float64) - b[0:3].astype(np.float64)
+    return float(np.sqrt((d * d).sum(0).mean()))
+
+
+def evaluate(corr, seed: int, sim_seconds: float = 1.0, frame_coarse: int = 5, log=None) -> dict:
+    """On held-out member `seed`: plain run at DT (reference), a second plain
+    run (its run-to-run noise: float-atomic order), the plain step at RATIO*DT
+    without correction, and the learned runner; RMS position error (cells)
+    against the reference after `sim_seconds`, and simulated seconds per
+    wall-clock second of each."""
+    sc = dambreak_family(seed)
+    dt = float(sc["cfg"].DT)
+    fine = int(round(sim_seconds / dt))
+    coarse = fine // RATIO
+    out = {"seed": seed, "dt": dt, "ratio": RATIO, "sim_seconds": sim_seconds, "fine_substeps": fine}
+
+    def timed(fn, e):
+        e.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        e.synchronize()
+        return time.perf_counter() - t0
+
+    ref = make_engine(sc, RESORT_EVERY=sc["resort_every"])
+    frame_fine = frame_coarse * RATIO
+    t_ref = timed(lambda: [ref.steps(frame_fine) for _ in range(fine // frame_fine)], ref)
+    ref_state = ref.getParticles()
+    ref.close()
+    out["plain_sim_s_per_s"] = sim_seconds / t_ref
+
+    again = make_engine(sc, RESORT_EVERY=sc["resort_every"])
+    for _ in range(fine // frame_fine):
+        again.steps(frame_fine)
+    out["noise_floor_rms"] = rms(again.getParticles(), ref_state)
+    again.close()
+
+    coarse_eng = make_engine(sc, RESORT_EVERY=max(1, sc["resort_every"] // RATIO))
+    coarse_eng.set_dt_scale(RATIO)
+    for _ in range(coarse // frame_coarse):
+        coarse_eng.steps(frame_coarse)
+    st = coarse_eng.getParticles()
+    out["plain_coarse_rms"] = rms(st, ref_state) if np.isfinite(st[0:3]).all() else float("inf")
+    coarse_eng.close()
+
+    le = make_engine(sc, RESORT_EVERY=max(1, sc["resort_every"] // RATIO))
+    runner = LearnedRunner(le, corr)
+    t_l = timed(lambda: [runner.frame(frame_coarse) for _ in range(coarse // frame_coarse)], le)
+    out["learned_rms"] = rms(le.getParticles(), ref_state)
+    out["learned_sim_s_per_s"] = sim_seconds / t_l
+    out["learned_speedup"] = out["learned_sim_s_per_s"] / out["plain_sim_s_per_s"]
+    out["frames"] = runner.frames
+    out["fallback_frames"] = runner.fallbacks
+    le.close()
+    if log:
+        log("evaluate: " + json.dumps(out))
+    return out
+
+
+def save(corr, path, mean, inv, scale, meta=None) -> None:
+    np.savez(path, params=corr.get_params(), mean=mean, inv_std=inv, out_scale=np.float32(scale),
+             meta=json.dumps(meta or {}))
+
+
+def load(path):
+    """A Corrector with the parameters and normalisation stored by save()."""
+    d = np.load(path, allow_pickle=False)
+    c = _pkg().Corrector(d["params"])
+    c.set_norm(d["mean"], d["inv_std"], float(d["out_scale"]))
+    return c, json.loads(str(d["meta"]))
