@@ -90,7 +90,7 @@ int mpm_config_derive(mpm_sim_config* cfg);
  * spaces removed, WATER=/OBJECT= as path;scale;x;y;z.  Every SimConfig key
  * that the reference reads with getInt/getFloat is required (missing key ->
  * MPM_EIO with the key named, where the reference exits).  `waters`/`objects`
- * may be NULL; *n_waters/*n_objects receive the entry counts, at most
+ * may be NULL; *n_waters and *n_objects receive the entry counts, at most
  * `cap` entries are written to each array. */
 int mpm_config_parse_file(const char* path, mpm_sim_config* cfg, mpm_config_object* waters,
                           int* n_waters, mpm_config_object* objects, int* n_objects, int cap);
@@ -285,6 +285,9 @@ int mpm_corrector_get_params(mpm_corrector* c, float* params, int n);
 /* Input normalisation x' = (x - mean) * inv_std (66 each) and the output
  * scale: correction = network output * out_scale. */
 int mpm_corrector_set_norm(mpm_corrector* c, const float* mean, const float* inv_std, float out_scale);
+/* Limit on |network output| (before out_scale) in the forward pass and in
+ * the engine [no limit]: a guard against extrapolation. */
+int mpm_corrector_set_clamp(mpm_corrector* c, float clamp);
 /* Y[n][3] = corrections of the n feature rows X[n][72] (host arrays). */
 int mpm_corrector_forward(mpm_corrector* c, const float* X, int n, float* Y);
 /* Training set on the device: features X[n][72], target corrections T[n][3]. */
@@ -301,9 +304,10 @@ int mpm_corrector_train(mpm_corrector* c, int steps, int batch, float lr, float*
  * every grid update. */
 int mpm_engine_set_corrector(mpm_engine* h, mpm_corrector* c);
 /* Training pairs at the current state: features of the active nodes of one
- * substep at ratio*DT (after its grid update) and, as targets, the mean grid
- * velocity of the same nodes over `ratio` plain substeps minus the coarse
- * velocity (nodes with mass in every plain substep only).  The engine ends
+ * substep at ratio*DT (after its grid update) and, as targets, the grid
+ * velocity of the same nodes after the last of `ratio` plain substeps minus
+ * the coarse velocity, for nodes with mass in every plain substep (0 for the
+ * others).  The engine ends
  * `ratio` plain substeps later.  Writes at most cap rows. */
 int mpm_engine_collect_pairs(mpm_engine* h, int ratio, float* X_out, float* T_out, int cap, int* count_out);
 /* Stage access for parity tests: the active nodes of the current grid

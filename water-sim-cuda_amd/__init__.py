@@ -147,6 +147,7 @@ def load_library() -> ctypes.CDLL:
         "mpm_corrector_set_params": (ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int]),
         "mpm_corrector_get_params": (ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int]),
         "mpm_corrector_set_norm": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, ctypes.c_float]),
+        "mpm_corrector_set_clamp": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
         "mpm_corrector_forward": (ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_float_p]),
         "mpm_corrector_load_dataset": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, ctypes.c_int]),
         "mpm_corrector_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_float_p, c_float_p]),
@@ -553,6 +554,9 @@ class Corrector:
         m = _f32(mean, (CORRECTOR_FEATURES,))
         s = _f32(inv_std, (CORRECTOR_FEATURES,))
         _check(self._lib.mpm_corrector_set_norm(self._h, _fptr(m), _fptr(s), float(out_scale)))
+
+    def set_clamp(self, clamp: float) -> None:
+        _check(self._lib.mpm_corrector_set_clamp(self._h, float(clamp)))
 
     @staticmethod
     def _rows(X) -> np.ndarray:

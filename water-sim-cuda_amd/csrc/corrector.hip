@@ -181,26 +181,37 @@ struct Norm {
     const float* mean;     // KIN
     const float* inv_std;  // KIN (0 for padding)
     float out_scale;       // correction = network output * out_scale
+    float clamp;           // |network output| limit of the forward pass
 };
 
 // Forward over rows [0, *count) (or n when count is null): Y[row][3] =
-// out_scale * MLP(X[row]).  One wave per tile, persistent over tiles.
-__global__ __launch_bounds__(64) void k_forward(const float* __restrict__ X, int n, const uint32_t* __restrict__ count,
-                                                Norm nm, const bf16x4* __restrict__ fr, const float* __restrict__ P,
-                                                float* __restrict__ Y) {
+// out_scale * clamp(MLP(X[row])).  Four waves per workgroup share the forward
+// weight fragments staged in LDS once; each wave loops over 32-row tiles.
+constexpr int FWD_FRAGS = B_L4;  // forward fragments F_L1..F_L4
+__global__ __launch_bounds__(256) void k_forward(const float* __restrict__ X, int n, const uint32_t* __restrict__ count,
+                                                 Norm nm, const bf16x4* __restrict__ fr, const float* __restrict__ P,
+                                                 float* __restrict__ Y) {
+    __shared__ bf16x4 wfr[FWD_FRAGS * 64];
+    for (int i = threadIdx.x; i < FWD_FRAGS * 64; i += blockDim.x) wfr[i] = fr[i];
+    __syncthreads();
     const int rows = count ? (int)min((uint32_t)n, *count) : n;
-    const int lane = threadIdx.x, hi = lane >> 5;
-    for (int tile = blockIdx.x; tile * 32 < rows; tile += gridDim.x) {
+    const int lane = threadIdx.x & 63, hi = lane >> 5, wave = threadIdx.x >> 6;
+    for (int tile = blockIdx.x * 4 + wave; tile * 32 < rows; tile += gridDim.x * 4) {
         const int row = tile * 32 + (lane & 31);
         const bool valid = row < rows;
         bf16x4 xb[9], h1[8], h2[8], h3[8];
         f32x16 a4;
         load_x(X, row, valid, nm.mean, nm.inv_std, hi, xb);
-        mlp_fwd(fr, P, xb, lane, h1, h2, h3, a4);
+        // Re-read the weight fragments from LDS in every tile (each is read
+        // right before its MFMA) instead of holding all 58 in registers.
+        asm volatile("" ::: "memory");
+        mlp_fwd(wfr, P, xb, lane, h1, h2, h3, a4);
         if (valid && hi == 0) {
-            Y[(size_t)row * 3 + 0] = a4[0] * nm.out_scale;
-            Y[(size_t)row * 3 + 1] = a4[1] * nm.out_scale;
-            Y[(size_t)row * 3 + 2] = a4[2] * nm.out_scale;
+            // guard: outputs beyond the clamp (in target-RMS units, far
+            // outside anything the training set holds) are limited
+            Y[(size_t)row * 3 + 0] = fminf(fmaxf(a4[0], -nm.clamp), nm.clamp) * nm.out_scale;
+            Y[(size_t)row * 3 + 1] = fminf(fmaxf(a4[1], -nm.clamp), nm.clamp) * nm.out_scale;
+            Y[(size_t)row * 3 + 2] = fminf(fmaxf(a4[2], -nm.clamp), nm.clamp) * nm.out_scale;
         }
     }
 }
@@ -420,7 +431,7 @@ __global__ __launch_bounds__(256) void k_active(const DevCfg* __restrict__ cfgs,
 //   62     GRAVITY * DT^2,  63  COMPRESSION * DT^2
 //   64     distance to the nearest padded wall / 8, clamped to [0, 1]
 //   65     y / SIZE_Y
-__device__ void node_features(const DevCfg& c, const GridBuf& V, const float* __restrict__ sdf, int x, int y, int z,
+__device__ __forceinline__ void node_features(const DevCfg& c, const GridBuf& V, const float* __restrict__ sdf, int x, int y, int z,
                               float* __restrict__ f) {
     const int g = gidx(c, x, y, z);
     const float dt = c.dt;
@@ -432,20 +443,19 @@ __device__ void node_features(const DevCfg& c, const GridBuf& V, const float* __
     float nx, ny, nz;
     node_normal(c, sdf, x, y, z, nx, ny, nz);
     f[5] = nx; f[6] = ny; f[7] = nz;
-    int o = 0;
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx, ++o) {
-                float mm = 0.0f, pm = 0.0f;
-                if (inb(c, x + dx, y + dy, z + dz)) {
-                    const int q = gidx(c, x + dx, y + dy, z + dz);
-                    mm = V.plane[0][q];
-                    const float vx = V.plane[1][q], vy = V.plane[2][q], vz = V.plane[3][q];
-                    pm = mm * sqrtf(vx * vx + vy * vy + vz * vz) * dt;
-                }
-                f[8 + o] = mm * 0.125f;
-                f[35 + o] = pm * 0.125f;
-            }
+#pragma unroll
+    for (int o = 0; o < 27; ++o) {
+        const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
+        float mm = 0.0f, pm = 0.0f;
+        if (inb(c, x + dx, y + dy, z + dz)) {
+            const int q = gidx(c, x + dx, y + dy, z + dz);
+            mm = V.plane[0][q];
+            const float vx = V.plane[1][q], vy = V.plane[2][q], vz = V.plane[3][q];
+            pm = mm * sqrtf(vx * vx + vy * vy + vz * vz) * dt;
+        }
+        f[8 + o] = mm * 0.125f;
+        f[35 + o] = pm * 0.125f;
+    }
     f[62] = c.gravity * dt * dt;
     f[63] = c.compression * dt * dt;
     const float p = (float)c.pad;
@@ -454,6 +464,7 @@ __device__ void node_features(const DevCfg& c, const GridBuf& V, const float* __
                           fminf((float)z - p, (float)(c.sz - 1) - p - (float)z)));
     f[64] = fminf(fmaxf(w * 0.125f, 0.0f), 1.0f);
     f[65] = (float)y / (float)c.sy;
+#pragma unroll
     for (int i = NF; i < KIN; ++i) f[i] = 0.0f;
 }
 
@@ -465,6 +476,53 @@ __global__ __launch_bounds__(256) void k_features(const DevCfg* __restrict__ cfg
         const int2 e = list[i];
         const DevCfg c = cfgs[e.y];
         node_features(c, V, sdf, e.x & 1023, (e.x >> 10) & 1023, e.x >> 20, X + (size_t)i * KIN);
+    }
+}
+
+// Features and forward pass fused: the corrections Y[row] of the listed
+// nodes, with each lane computing its node's feature row in registers (the
+// two lanes of a node both compute it and keep their halves), so no feature
+// rows travel through memory.  Weight fragments are staged in LDS.
+__global__ __launch_bounds__(256) void k_correct(const DevCfg* __restrict__ cfgs, GridBuf V,
+                                                 const float* __restrict__ sdf, const int2* __restrict__ list,
+                                                 const uint32_t* __restrict__ count, int cap, Norm nm,
+                                                 const bf16x4* __restrict__ fr, const float* __restrict__ P,
+                                                 float* __restrict__ Y) {
+    __shared__ bf16x4 wfr[FWD_FRAGS * 64];
+    for (int i = threadIdx.x; i < FWD_FRAGS * 64; i += blockDim.x) wfr[i] = fr[i];
+    __syncthreads();
+    const int rows = (int)min(*count, (uint32_t)cap);
+    const int lane = threadIdx.x & 63, hi = lane >> 5, wave = threadIdx.x >> 6;
+    for (int tile = blockIdx.x * 4 + wave; tile * 32 < rows; tile += gridDim.x * 4) {
+        const int row = tile * 32 + (lane & 31);
+        const bool valid = row < rows;
+        float f[KIN];
+        if (valid) {
+            const int2 e = list[row];
+            node_features(cfgs[e.y], V, sdf, e.x & 1023, (e.x >> 10) & 1023, e.x >> 20, f);
+        } else {
+#pragma unroll
+            for (int i = 0; i < KIN; ++i) f[i] = nm.mean[i];  // normalises to 0
+        }
+        bf16x4 xb[9], h1[8], h2[8], h3[8];
+#pragma unroll
+        for (int kb = 0; kb < 9; ++kb) {
+            float v[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int k0 = kb * 8 + i, k1 = k0 + 4;
+                v[i] = hi ? (f[k1] - nm.mean[k1]) * nm.inv_std[k1] : (f[k0] - nm.mean[k0]) * nm.inv_std[k0];
+            }
+            xb[kb] = pack4(v[0], v[1], v[2], v[3]);
+        }
+        f32x16 a4;
+        asm volatile("" ::: "memory");  // weight fragments are re-read from LDS per tile
+        mlp_fwd(wfr, P, xb, lane, h1, h2, h3, a4);
+        if (valid && hi == 0) {
+            Y[(size_t)row * 3 + 0] = fminf(fmaxf(a4[0], -nm.clamp), nm.clamp) * nm.out_scale;
+            Y[(size_t)row * 3 + 1] = fminf(fmaxf(a4[1], -nm.clamp), nm.clamp) * nm.out_scale;
+            Y[(size_t)row * 3 + 2] = fminf(fmaxf(a4[2], -nm.clamp), nm.clamp) * nm.out_scale;
+        }
     }
 }
 
@@ -483,8 +541,8 @@ __global__ void k_apply(const DevCfg* __restrict__ cfgs, GridBuf V, const int2* 
 }
 
 // Training pairs: the velocities of the listed nodes (after a grid update),
-// copied (mode 0) or accumulated with a count of the substeps in which the
-// node had mass (mode 1).
+// copied (mode 0), accumulated (mode 1) or copied (mode 2), modes 1 and 2
+// with a count of the calls in which the node had mass.
 __global__ void k_gather_vel(const DevCfg* __restrict__ cfgs, GridBuf V, const int2* __restrict__ list,
                              const uint32_t* __restrict__ count, int cap, float* __restrict__ out,
                              float* __restrict__ hits, int mode) {
@@ -495,12 +553,12 @@ __global__ void k_gather_vel(const DevCfg* __restrict__ cfgs, GridBuf V, const i
         const int g = gidx(c, e.x & 1023, (e.x >> 10) & 1023, e.x >> 20);
         const bool live = V.plane[0][g] > 1e-9f;
         const float vx = live ? V.plane[1][g] : 0.0f, vy = live ? V.plane[2][g] : 0.0f, vz = live ? V.plane[3][g] : 0.0f;
-        if (mode == 0) {
-            out[(size_t)i * 3] = vx; out[(size_t)i * 3 + 1] = vy; out[(size_t)i * 3 + 2] = vz;
-        } else {
+        if (mode == 1) {
             out[(size_t)i * 3] += vx; out[(size_t)i * 3 + 1] += vy; out[(size_t)i * 3 + 2] += vz;
-            hits[i] += live ? 1.0f : 0.0f;
+        } else {
+            out[(size_t)i * 3] = vx; out[(size_t)i * 3 + 1] = vy; out[(size_t)i * 3 + 2] = vz;
         }
+        if (mode != 0) hits[i] += live ? 1.0f : 0.0f;
     }
 }
 
@@ -558,9 +616,16 @@ void launch_corr_features(const DevCfg* cfgs, GridBuf V, const float* sdf, const
 }
 
 void launch_corr_forward(const CorrState& cs, const float* X, int cap, const uint32_t* count, float* Y, hipStream_t s) {
-    Norm nm{cs.mean, cs.inv_std, cs.out_scale};
-    hipLaunchKernelGGL(k_forward, dim3(grid_blocks(cap, 32, 8192)), dim3(64), 0, s, X, cap, count, nm,
+    Norm nm{cs.mean, cs.inv_std, cs.out_scale, cs.out_clamp};
+    hipLaunchKernelGGL(k_forward, dim3(grid_blocks(cap, 128, 1024)), dim3(256), 0, s, X, cap, count, nm,
                        (const bf16x4*)cs.frags, cs.params, Y);
+}
+
+void launch_corr_correct(const DevCfg* cfgs, GridBuf V, const float* sdf, const int2* list, const uint32_t* count,
+                         int cap, const CorrState& cs, float* Y, hipStream_t s) {
+    Norm nm{cs.mean, cs.inv_std, cs.out_scale, cs.out_clamp};
+    hipLaunchKernelGGL(k_correct, dim3(grid_blocks(cap, 128, 1024)), dim3(256), 0, s, cfgs, V, sdf, list, count, cap,
+                       nm, (const bf16x4*)cs.frags, cs.params, Y);
 }
 
 void launch_corr_apply(const DevCfg* cfgs, GridBuf V, const int2* list, const uint32_t* count, int cap, const float* Y,
@@ -646,14 +711,15 @@ int mpm_corrector_create(mpm_corrector** out) {
     std::vector<float> zero(CORR_PARAMS, 0.0f), mean(KIN, 0.0f), inv(KIN, 0.0f);
     for (int i = 0; i < NF; ++i) inv[i] = 1.0f;
     c->st.out_scale = 1.0f;
-    hipMemcpy(c->st.params, zero.data(), sizeof(float) * CORR_PARAMS, hipMemcpyHostToDevice);
-    hipMemcpy(c->grad, zero.data(), sizeof(float) * CORR_PARAMS, hipMemcpyHostToDevice);
-    hipMemcpy(c->adam_m, zero.data(), sizeof(float) * CORR_PARAMS, hipMemcpyHostToDevice);
-    hipMemcpy(c->adam_v, zero.data(), sizeof(float) * CORR_PARAMS, hipMemcpyHostToDevice);
-    hipMemcpy(c->st.mean, mean.data(), sizeof(float) * KIN, hipMemcpyHostToDevice);
-    hipMemcpy(c->st.inv_std, inv.data(), sizeof(float) * KIN, hipMemcpyHostToDevice);
+    c->st.out_clamp = 1e30f;
+    hipError_t e = hipMemcpy(c->st.params, zero.data(), sizeof(float) * CORR_PARAMS, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->grad, zero.data(), sizeof(float) * CORR_PARAMS, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->adam_m, zero.data(), sizeof(float) * CORR_PARAMS, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->adam_v, zero.data(), sizeof(float) * CORR_PARAMS, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->st.mean, mean.data(), sizeof(float) * KIN, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->st.inv_std, inv.data(), sizeof(float) * KIN, hipMemcpyHostToDevice);
     corr_pack(c->st.params, c->st.frags, c->stream);
-    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    if (e != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
         mpm_corrector_destroy(c);
         return cfail(MPM_EHIP, "corrector init failed");
     }
@@ -663,10 +729,11 @@ int mpm_corrector_create(mpm_corrector** out) {
 
 int mpm_corrector_destroy(mpm_corrector* c) {
     if (!c) return MPM_OK;
-    if (c->stream) hipStreamSynchronize(c->stream);
-    hipFree(c->st.params); hipFree(c->st.frags); hipFree(c->st.mean); hipFree(c->st.inv_std);
-    hipFree(c->grad); hipFree(c->adam_m); hipFree(c->adam_v); hipFree(c->loss); hipFree(c->dX); hipFree(c->dT);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* q : {(void*)c->st.params, c->st.frags, (void*)c->st.mean, (void*)c->st.inv_std, (void*)c->grad,
+                    (void*)c->adam_m, (void*)c->adam_v, (void*)c->loss, (void*)c->dX, (void*)c->dT})
+        if (q) (void)hipFree(q);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MPM_OK;
 }
@@ -690,6 +757,12 @@ int mpm_corrector_get_params(mpm_corrector* c, float* params, int n) {
     return MPM_OK;
 }
 
+int mpm_corrector_set_clamp(mpm_corrector* c, float clamp) {
+    if (!c || !(clamp > 0.0f)) return cfail(MPM_EINVAL, "clamp must be positive");
+    c->st.out_clamp = clamp;
+    return MPM_OK;
+}
+
 int mpm_corrector_set_norm(mpm_corrector* c, const float* mean, const float* inv_std, float out_scale) {
     if (!c || !mean || !inv_std || !(out_scale > 0.0f)) return cfail(MPM_EINVAL, "bad normalisation");
     std::vector<float> m(KIN, 0.0f), s(KIN, 0.0f);
@@ -710,23 +783,23 @@ int mpm_corrector_forward(mpm_corrector* c, const float* X, int n, float* Y) {
     float *dx = nullptr, *dy = nullptr;
     int rc;
     if ((rc = alloc(&dx, (size_t)n * KIN)) || (rc = alloc(&dy, (size_t)n * 3))) {
-        hipFree(dx);
+        (void)hipFree(dx);
         return rc;
     }
-    hipMemcpyAsync(dx, X, sizeof(float) * KIN * (size_t)n, hipMemcpyHostToDevice, c->stream);
-    launch_corr_forward(c->st, dx, n, nullptr, dy, c->stream);
-    hipMemcpyAsync(Y, dy, sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToHost, c->stream);
-    hipError_t e = hipStreamSynchronize(c->stream);
-    hipFree(dx);
-    hipFree(dy);
+    hipError_t e = hipMemcpyAsync(dx, X, sizeof(float) * KIN * (size_t)n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) launch_corr_forward(c->st, dx, n, nullptr, dy, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(Y, dy, sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dx);
+    (void)hipFree(dy);
     if (e != hipSuccess) return cfail(MPM_EHIP, std::string("corrector forward: ") + hipGetErrorString(e));
     return MPM_OK;
 }
 
 int mpm_corrector_load_dataset(mpm_corrector* c, const float* X, const float* T, int n) {
     if (!c || n < 1 || !X || !T) return cfail(MPM_EINVAL, "bad dataset");
-    hipFree(c->dX);
-    hipFree(c->dT);
+    if (c->dX) (void)hipFree(c->dX);
+    if (c->dT) (void)hipFree(c->dT);
     c->dX = c->dT = nullptr;
     c->nrows = 0;
     int rc;
@@ -748,7 +821,7 @@ int mpm_corrector_gradient(mpm_corrector* c, int row0, int n, float* grad_out, f
         if ((rc = alloc(&c->loss, 1))) return rc;
         c->loss_cap = 1;
     }
-    Norm nm{c->st.mean, c->st.inv_std, c->st.out_scale};
+    Norm nm{c->st.mean, c->st.inv_std, c->st.out_scale, c->st.out_clamp};
     hipLaunchKernelGGL(k_zero_f32, dim3((CORR_PARAMS + 255) / 256), dim3(256), 0, c->stream, c->grad, CORR_PARAMS);
     hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, c->stream, c->loss, 1);
     hipLaunchKernelGGL(k_train, dim3(grid_blocks(n, 32, 1024)), dim3(64), 0, c->stream, c->dX, c->dT, row0, n,
@@ -770,14 +843,14 @@ int mpm_corrector_train(mpm_corrector* c, int steps, int batch, float lr, float*
     if (!c || steps < 1 || batch < 32 || !(lr > 0.0f)) return cfail(MPM_EINVAL, "bad training arguments");
     if (!c->nrows) return cfail(MPM_ESTATE, "no dataset loaded");
     if (c->loss_cap < steps) {
-        hipFree(c->loss);
+        if (c->loss) (void)hipFree(c->loss);
         c->loss = nullptr;
         int rc;
         if ((rc = alloc(&c->loss, (size_t)steps))) return rc;
         c->loss_cap = steps;
     }
     hipLaunchKernelGGL(k_zero_f32, dim3((steps + 255) / 256), dim3(256), 0, c->stream, c->loss, steps);
-    Norm nm{c->st.mean, c->st.inv_std, c->st.out_scale};
+    Norm nm{c->st.mean, c->st.inv_std, c->st.out_scale, c->st.out_clamp};
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     for (int s = 0; s < steps; ++s) {
         const int row0 = (int)(((int64_t)s * batch + c->adam_t * 7919) % c->nrows);

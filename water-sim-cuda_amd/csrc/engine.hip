@@ -371,12 +371,11 @@ void run_grid_update(mpm_engine* h) {
         StageTimer t(h, ST_CORR);
         launch_set_u32(corr_count(h), 0, 0u, h->stream);
         launch_corr_active(h->d_cfgs, h->d_block_scene, V, h->nfb, h->clist, h->ccap, corr_count(h), h->stream);
-        launch_corr_features(h->d_cfgs, V, h->sdf, h->clist, corr_count(h), h->ccap, h->cX, h->stream);
-        launch_corr_forward(*h->corr, h->cX, h->ccap, corr_count(h), h->cY, h->stream);
+        launch_corr_correct(h->d_cfgs, V, h->sdf, h->clist, corr_count(h), h->ccap, *h->corr, h->cY, h->stream);
         launch_corr_apply(h->d_cfgs, V, h->clist, corr_count(h), h->ccap, h->cY, h->stream);
     }
     if (h->pair_mode)
-        launch_corr_gather_vel(h->d_cfgs, V, h->clist, corr_count(h), h->ccap, h->cacc, h->chits, 1, h->stream);
+        launch_corr_gather_vel(h->d_cfgs, V, h->clist, corr_count(h), h->ccap, h->cacc, h->chits, 2, h->stream);
 }
 
 int zero_grid(mpm_engine* h, int which) {
@@ -959,7 +958,7 @@ int mpm_engine_collect_pairs(mpm_engine* h, int ratio, float* X_out, float* T_ou
     launch_corr_features(h->d_cfgs, V, h->sdf, h->clist, corr_count(h), h->ccap, h->cX, h->stream);
     launch_corr_gather_vel(h->d_cfgs, V, h->clist, corr_count(h), h->ccap, h->cY, nullptr, 0, h->stream);
     if ((rc = upload_dt(h, scale0))) return rc;
-    // back to the saved particles; `ratio` plain substeps accumulate the grid
+    // back to the saved particles; `ratio` plain substeps record the grid
     // velocities of the same nodes
     if ((rc = mpm_engine_restore(h))) return rc;
     launch_fill_f32(h->cacc, (size_t)h->ccap * 3, 0.0f, h->stream);
@@ -978,13 +977,18 @@ int mpm_engine_collect_pairs(mpm_engine* h, int ratio, float* X_out, float* T_ou
         HIPCHK(hipMemcpyAsync(hits.data(), h->chits, hits.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
     }
-    // keep nodes that had mass in every fine substep; target = mean fine
-    // velocity - coarse velocity
+    // target = grid velocity after the last fine substep's grid update -
+    // coarse velocity, for nodes that had mass in every fine substep; no
+    // correction (0) for the others (surface nodes the fine steps left or
+    // reached late).  G2P carries the corrected grid velocity into the
+    // particles' velocities, which persist into the next substep, so the
+    // correction aims at the velocity at the end of the interval.
     int w = 0;
     for (int i = 0; i < rows && w < cap; ++i) {
-        if (hits[i] < (float)ratio) continue;
+        const bool full = hits[i] >= (float)ratio;
         std::memcpy(X_out + (size_t)w * CORR_STRIDE, X.data() + (size_t)i * CORR_STRIDE, CORR_STRIDE * sizeof(float));
-        for (int j = 0; j < 3; ++j) T_out[(size_t)w * 3 + j] = acc[(size_t)i * 3 + j] / (float)ratio - vc[(size_t)i * 3 + j];
+        for (int j = 0; j < 3; ++j)
+            T_out[(size_t)w * 3 + j] = full ? acc[(size_t)i * 3 + j] - vc[(size_t)i * 3 + j] : 0.0f;
         ++w;
     }
     *count_out = w;

@@ -127,6 +127,7 @@ struct CorrState {
     float* mean;      // CORR_STRIDE
     float* inv_std;   // CORR_STRIDE
     float out_scale;  // correction = output * out_scale
+    float out_clamp;  // |output| limit applied by the forward pass
 };
 
 void corr_pack(const float* params, void* frags, hipStream_t s);
@@ -137,10 +138,13 @@ void launch_corr_active(const DevCfg* cfgs, const uint16_t* block_scene, GridBuf
 void launch_corr_features(const DevCfg* cfgs, GridBuf V, const float* sdf, const int2* list, const uint32_t* count,
                           int cap, float* X, hipStream_t s);
 void launch_corr_forward(const CorrState& cs, const float* X, int cap, const uint32_t* count, float* Y, hipStream_t s);
+// Features + forward pass of the listed nodes in one kernel: corrections Y.
+void launch_corr_correct(const DevCfg* cfgs, GridBuf V, const float* sdf, const int2* list, const uint32_t* count,
+                         int cap, const CorrState& cs, float* Y, hipStream_t s);
 void launch_corr_apply(const DevCfg* cfgs, GridBuf V, const int2* list, const uint32_t* count, int cap, const float* Y,
                        hipStream_t s);
-// Velocities of the listed nodes: copied (mode 0) or accumulated with a count
-// of the calls in which the node had mass (mode 1).
+// Velocities of the listed nodes: copied (mode 0), accumulated (mode 1) or
+// copied (mode 2); modes 1 and 2 count the calls in which the node had mass.
 void launch_corr_gather_vel(const DevCfg* cfgs, GridBuf V, const int2* list, const uint32_t* count, int cap, float* out,
                             float* hits, int mode, hipStream_t s);
 // out3 += {particles below SDF -depth, sum |det F - 1|, non-finite particles}.

@@ -5,8 +5,11 @@ The plain MLS-MPM substep at DT is the ground truth.  A corrected substep at
 RATIO*DT runs P2G and the grid update at the large step, adds the network's
 per-node velocity correction, and runs G2P.  Training pairs come from plain
 runs: at a state S, the features of the active nodes after one RATIO*DT grid
-update, and as targets the mean grid velocity of the same nodes over RATIO
-plain substeps from S minus the coarse velocity (Engine.collect_pairs).
+update, and as targets the grid velocity of the same nodes after the last of
+RATIO plain substeps from S minus the coarse velocity (Engine.collect_pairs).
+The target is the end-of-interval velocity, not the interval's mean, because
+G2P turns the corrected grid velocity into the particles' velocities, which
+persist: a mean-velocity correction is re-applied every substep and drifts.
 
 The plain step is the fallback: each frame of corrected substeps starts from
 a device snapshot, and when a check trips afterwards -- particles deeper than
@@ -26,6 +29,7 @@ RATIO = 4
 PEN_DEPTH = 0.5      # cells
 PEN_FRACTION = 1e-3  # of the particles
 DRIFT_MAX = 0.05     # mean |det F - 1|
+OUT_CLAMP = 8.0      # |network output| limit, in target-RMS units
 
 
 def _pkg():
@@ -166,7 +170,7 @@ def rms(a: np.ndarray, b: np.ndarray) -> float:
     return float(np.sqrt((d * d).sum(0).mean()))
 
 
-def evaluate(corr, seed: int, sim_seconds: float = 1.0, frame_coarse: int = 5, log=None) -> dict:
+def evaluate(corr, seed: int, sim_seconds: float = 1.0, frame_coarse: int = 10, log=None) -> dict:
     """On held-out member `seed`: plain run at DT (reference), a second plain
     run (its run-to-run noise: float-atomic order), the plain step at RATIO*DT
     without correction, and the learned runner; RMS position error (cells)
@@ -200,8 +204,8 @@ def evaluate(corr, seed: int, sim_seconds: float = 1.0, frame_coarse: int = 5, l
 
     coarse_eng = make_engine(sc, RESORT_EVERY=max(1, sc["resort_every"] // RATIO))
     coarse_eng.set_dt_scale(RATIO)
-    for _ in range(coarse // frame_coarse):
-        coarse_eng.steps(frame_coarse)
+    t_c = timed(lambda: [coarse_eng.steps(frame_coarse) for _ in range(coarse // frame_coarse)], coarse_eng)
+    out["plain_coarse_sim_s_per_s"] = sim_seconds / t_c
     st = coarse_eng.getParticles()
     out["plain_coarse_rms"] = rms(st, ref_state) if np.isfinite(st[0:3]).all() else float("inf")
     coarse_eng.close()
@@ -230,4 +234,5 @@ def load(path):
     d = np.load(path, allow_pickle=False)
     c = _pkg().Corrector(d["params"])
     c.set_norm(d["mean"], d["inv_std"], float(d["out_scale"]))
+    c.set_clamp(OUT_CLAMP)
     return c, json.loads(str(d["meta"]))
