@@ -75,6 +75,8 @@ def main() -> int:
     ap.add_argument("--tune-cache", default=None, help="JSON cache file for --autotune")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "g2p2g_channel256_traffic.txt"),
                     help="rocprofv3-derived HBM bytes per launch of the fused kernel (key: value text)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-corrector evaluation")
+    ap.add_argument("--learned-seconds", type=float, default=1.0, help="simulated seconds of the learned evaluation")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the calibration copy of BYTES and print its bandwidth")
     args = ap.parse_args()
@@ -206,6 +208,30 @@ def main() -> int:
         log(f"cpu baseline: {cpu}")
     eng.close()
 
+    learned = None
+    if not args.no_learned and not batch:
+        # The learned 4x-DT form against the plain step on a held-out member of
+        # the 128^3 dam-break family (weights trained by train_corrector.py).
+        from importlib import import_module
+
+        L = import_module("water-sim-cuda_amd.learned")
+        wpath = Path(mpm.__file__).parent / "corrector_dambreak128.npz"
+        corr, meta = L.load(wpath)
+        ev = L.evaluate(corr, 1000, sim_seconds=args.learned_seconds)
+        corr.close()
+        learned = {"workload": "dambreak128 held-out seed 1000, 1 simulated second",
+                   "ratio": ev["ratio"], "plain_dt": ev["dt"],
+                   "plain_sim_s_per_s": round(ev["plain_sim_s_per_s"], 4),
+                   "learned_sim_s_per_s": round(ev["learned_sim_s_per_s"], 4),
+                   "speedup": round(ev["learned_speedup"], 4),
+                   "learned_rms_cells": round(ev["learned_rms"], 5),
+                   "plain_coarse_sim_s_per_s": round(ev["plain_coarse_sim_s_per_s"], 4),
+                   "plain_coarse_rms_cells": round(ev["plain_coarse_rms"], 5),
+                   "noise_floor_rms_cells": ev["noise_floor_rms"],
+                   "fallback_frames": ev["fallback_frames"], "frames": ev["frames"],
+                   "training": {k: meta[k] for k in ("train_seeds", "samples", "epochs", "loss_last")}}
+        log(f"learned: {learned}")
+
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -226,6 +252,7 @@ def main() -> int:
                    "description": sc["description"], "state_finite": finite},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "learned": learned,
         "stage_ms_per_launch": {k: (round(v[0] / v[1], 5) if v[1] else None) for k, v in st.items()},
     }
     print(json.dumps(out), flush=True)

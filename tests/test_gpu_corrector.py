@@ -220,3 +220,41 @@ def test_fallback_rewinds_to_plain_step(gpu):
     pa, pb = a.getParticles(), b.getParticles()
     assert float(np.abs(pa[0:3] - pb[0:3]).max()) < 24e-4
     a.close(); b.close(); c.close()
+
+
+def test_trained_corrector_reduces_one_step_error(gpu):
+    """The committed dam-break corrector on a held-out seed, pairs taken as in
+    training (4xDT step at substeps 0, 500, 1000, 1500 of a plain DT run): the
+    pooled RMS gap between the 4xDT grid velocity and the plain-DT velocity
+    4 substeps later (the quantity it is trained on) shrinks by at least a
+    quarter with the network's correction (measured: about 1.6x; on its
+    training seeds the loss implies about 5x, so it generalises only in part).  Per-time figures are printed:
+    the gap is concentrated in few nodes (dam front, collider); in the slowly
+    compressing bulk it is below the network's residual."""
+    from importlib import import_module
+    from pathlib import Path
+
+    learned = import_module("water-sim-cuda_amd.learned")
+    wpath = Path(gpu.__file__).parent / "corrector_dambreak128.npz"
+    corr, meta = learned.load(wpath)
+    sc = learned.dambreak_family(1000)
+    e = learned.make_engine(sc, RESORT_EVERY=sc["resort_every"])
+    done = 0
+    Ts, Ys = [], []
+    for t in (0, 500, 1000, 1500):
+        if t > done:
+            e.steps(t - done)
+        X, T = e.collect_pairs(learned.RATIO, 400000)
+        done = t + learned.RATIO
+        Y = corr.forward(X)
+        Ts.append(T)
+        Ys.append(Y)
+        say(f"trained corrector at substep {t}: gap rms {float(np.sqrt((T.astype(np.float64) ** 2).mean())):.3e} -> "
+            f"{float(np.sqrt(((T - Y).astype(np.float64) ** 2).mean())):.3e} ({len(X)} nodes)")
+    T = np.concatenate(Ts).astype(np.float64)
+    Y = np.concatenate(Ys).astype(np.float64)
+    before = float(np.sqrt((T ** 2).mean()))
+    after = float(np.sqrt(((T - Y) ** 2).mean()))
+    say(f"trained corrector pooled: {before:.3e} -> {after:.3e}")
+    assert after < 0.75 * before
+    e.close(); corr.close()

@@ -36,6 +36,19 @@ struct SceneTable {
     int local_bits;     // bits of a scene-local base-cell key; the scene id sits above
 };
 
+// A wave-uniform struct read from memory, moved into scalar registers field by
+// field (the loads themselves stay vector loads: the kernels also write
+// memory, so the compiler cannot use the scalar cache for them).
+template <class T> __device__ __forceinline__ T uniform(const T& v) {
+    static_assert(sizeof(T) % 4 == 0, "32-bit fields");
+    T r;
+    const int* src = reinterpret_cast<const int*>(&v);
+    int* dst = reinterpret_cast<int*>(&r);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    return r;
+}
+
 __device__ __forceinline__ int scene_of(const SceneTable& t, int i) {
     int lo = 0, hi = t.nscenes - 1;
     while (lo < hi) {

@@ -21,7 +21,7 @@ __global__ __launch_bounds__(256) void k_grid_update(const DevCfg* __restrict__ 
     const bool fm = m.flags[blk] != 0;
     const bool fv = clear_v && v.flags[blk] != 0;
     if (!fm && !fv) return;
-    const DevCfg c = cfgs[block_scene[blk]];
+    const DevCfg c = uniform(cfgs[block_scene[blk]]);
     const int lb = blk - c.foff;
     const int bx = lb % c.fbx, by = (lb / c.fbx) % c.fby, bz = lb / (c.fbx * c.fby);
     const int x = bx * 4 + (lane & 3), y = by * 4 + ((lane >> 2) & 3), z = bz * 4 + (lane >> 4);

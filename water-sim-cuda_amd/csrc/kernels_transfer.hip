@@ -273,7 +273,8 @@ __device__ __forceinline__ bool stencil_in_tile(int lx, int ly, int lz) {
 }
 
 template <int MODE>
-__global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
+// At most 168 VGPRs: three waves per SIMD (LDS allows about 13 chunks per CU).
+__global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a) {
     constexpr bool DO_G2P = (MODE & XFER_G2P) != 0;
     constexpr bool DO_P2G = (MODE & XFER_P2G) != 0;
     // Workgroups are dispatched to the 8 XCDs round-robin; chunk ranges are
@@ -283,10 +284,12 @@ __global__ __launch_bounds__(TRANSFER_THREADS) void k_transfer(TransferArgs a) {
     const int per = (int)((nch + 7u) >> 3);
     const int xcd = blockIdx.x & 7, nl = gridDim.x >> 3;
     for (int t = blockIdx.x >> 3; t < per; t += nl) {
-    const int ci = xcd * per + t;
+    const int ci = __builtin_amdgcn_readfirstlane(xcd * per + t);
     if (ci >= (int)nch) break;
-    const ChunkDesc d = a.chunks[ci];
-    const DevCfg c = a.cfgs[d.scene];
+    // The chunk's descriptor and its scene's config are wave-uniform: moved
+    // to scalar registers once per chunk, they take no vector registers.
+    const ChunkDesc d = uniform(a.chunks[ci]);
+    const DevCfg c = uniform(a.cfgs[d.scene]);
 
     __shared__ float4 vt[DO_G2P ? TILE_SLOTS : 1];
     __shared__ float4 ar[DO_P2G ? TILE_SLOTS : 1];  // mass, momentum x, y, z
