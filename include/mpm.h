@@ -151,6 +151,10 @@ int mpm_batch_scene_info(mpm_engine* h, int scene, int64_t* out4);
  * buffer, sets F's diagonal to 1, V = RESOLUTION^3, m = 1, zeroes the grid and
  * performs the initial Morton sort.  n = 0 is allowed (steps are no-ops). */
 int mpm_engine_init_particles(mpm_engine* h, int n, const float* soa26);
+/* Loads a saved 26*n state (get_particles output) as it is: like
+ * init_particles but without initFMatrices, so F, v and C continue (no
+ * reference counterpart: checkpoint/resume). */
+int mpm_engine_load_state(mpm_engine* h, int n, const float* soa26);
 
 /* Engine::initSDF + makeSDF (engine.cu:105-114, kernels.cu:282-300): builds the
  * node SDF from nt triangles (nt = 0 gives +1e15 everywhere, as the reference). */
@@ -264,6 +268,39 @@ int mpm_mesh_u_channel(const float* lo, const float* hi, float wall, float seg, 
                        int* ntris);
 /* Axis-aligned box mesh lo..hi (12 triangles, outward winding). */
 int mpm_mesh_box(const float* lo, const float* hi, mpm_triangle* tris_out, int* ntris);
+
+/* ---- scene front-end: OBJ meshes and config.txt (Simulation.cpp:4-38) ----
+ * Host-only; the results are the particle buffer and triangles that the
+ * reference's Simulation constructor hands to Engine::init and initSDF.
+ * Two-call pattern: call with cap 0 to get the count. */
+
+/* ObjLoader::loadObj (ObjLoader.cpp:9-33, tinyobjloader with its default
+ * triangulation) + VoxelEngine::extractTriangles (VoxelEngine.cpp:858-899):
+ * the triangles of an OBJ file; quads are split along the shorter diagonal. */
+int mpm_obj_triangles(const char* path, mpm_triangle* out, int cap, int* count);
+/* VoxelEngine::voxelize (VoxelEngine.cpp:120-243): centres of the cells of a
+ * (longest side / resolution) grid over the mesh's bounding box that a
+ * triangle overlaps (separating-axis test); xyz planes of cap floats. */
+int mpm_voxelize_obj(const char* path, float resolution, float* xyz, int64_t cap, int64_t* count);
+/* Simulation::Prepare_object (Simulation.cpp:80-92): voxelize(mesh, 100) and
+ * VoxelEngine::normalize (VoxelEngine.cpp:245-494) to normalize_size, scale,
+ * displacement, lattice step `lattice` (RESOLUTION as an integer), 5x5x5
+ * dilation, unique points in (x, y, z) order; xyz planes of cap floats. */
+int mpm_obj_water(const char* path, float normalize_size, float scale, const float* displacement, int lattice,
+                  float* xyz, int64_t cap, int64_t* count);
+/* Simulation::Prepare_triangles (Simulation.cpp:94-120): the mesh's
+ * triangles through VoxelEngine::normalize (VoxelEngine.cpp:601-827),
+ * triangles with no vertex inside [0, normalize_size]^3 dropped. */
+int mpm_obj_collider(const char* path, float normalize_size, float scale, const float* displacement,
+                     mpm_triangle* out, int cap, int* count);
+/* The Simulation constructor's scene (Simulation.cpp:4-38): parses the
+ * config file, builds every WATER entry (merged in file order) into the
+ * 26 x n buffer of VoxelPipeline::process (positions, zeros, VOL = MASS = 1)
+ * and every OBJECT entry into triangles.  Model paths are opened as given
+ * (relative to the working directory, as the reference does), else relative
+ * to the config file's directory.  The buffer is written when cap >= n. */
+int mpm_scene_from_config(const char* config_path, mpm_sim_config* cfg, float* soa26, int64_t cap, int64_t* n,
+                          mpm_triangle* tris, int tri_cap, int* ntris);
 
 /* ---- learned grid-velocity corrector (no reference counterpart) ----
  * An MLP 66 -> 64 -> 64 -> 64 -> 3 (ReLU; BF16 weights and activations, FP32

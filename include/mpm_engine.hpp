@@ -49,6 +49,8 @@ public:
 
     // Engine::init (engine.cu:45-52): n particles, 26*n floats, attribute-major.
     void init(int n, const float* h_buffer) { check(mpm_engine_init_particles(h_, n, h_buffer)); }
+    // Resume from a saved state (getParticles output): no initFMatrices.
+    void loadState(int n, const float* state) { check(mpm_engine_load_state(h_, n, state)); }
     // Engine::initSDF (engine.cu:105-114)
     void initSDF(const std::vector<Triangle>& triangles) {
         check(mpm_engine_init_sdf_from_triangles(h_, triangles.data(), (int)triangles.size()));

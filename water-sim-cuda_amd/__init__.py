@@ -121,6 +121,7 @@ def load_library() -> ctypes.CDLL:
         "mpm_engine_set_option": (ctypes.c_int, [eng, ctypes.c_char_p, ctypes.c_int64]),
         "mpm_engine_get_option": (ctypes.c_int, [eng, ctypes.c_char_p, c_i64_p]),
         "mpm_engine_init_particles": (ctypes.c_int, [eng, ctypes.c_int, c_float_p]),
+        "mpm_engine_load_state": (ctypes.c_int, [eng, ctypes.c_int, c_float_p]),
         "mpm_engine_init_sdf_from_triangles": (ctypes.c_int, [eng, c_float_p, ctypes.c_int]),
         "mpm_engine_set_sdf": (ctypes.c_int, [eng, c_float_p]),
         "mpm_engine_get_sdf": (ctypes.c_int, [eng, c_float_p]),
@@ -161,6 +162,14 @@ def load_library() -> ctypes.CDLL:
         "mpm_batch_get_grid": (ctypes.c_int, [eng, ctypes.c_int, c_float_p]),
         "mpm_batch_scene_info": (ctypes.c_int, [eng, ctypes.c_int, c_i64_p]),
         "mpm_probe_copy": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, c_dbl_p]),
+        "mpm_obj_triangles": (ctypes.c_int, [ctypes.c_char_p, c_float_p, ctypes.c_int, c_int_p]),
+        "mpm_voxelize_obj": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_float, c_float_p, ctypes.c_int64, c_i64_p]),
+        "mpm_obj_water": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_float, ctypes.c_float, c_float_p, ctypes.c_int,
+                                         c_float_p, ctypes.c_int64, c_i64_p]),
+        "mpm_obj_collider": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_float, ctypes.c_float, c_float_p, c_float_p,
+                                            ctypes.c_int, c_int_p]),
+        "mpm_scene_from_config": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(SimConfig), c_float_p, ctypes.c_int64,
+                                                 c_i64_p, c_float_p, ctypes.c_int, c_int_p]),
         "mpm_scene_fill": (
             ctypes.c_int,
             [ctypes.c_int, c_float_p, c_float_p, ctypes.c_uint64, c_float_p, ctypes.c_int64, c_i64_p],
@@ -278,6 +287,66 @@ def mesh_box(lo, hi) -> np.ndarray:
     return _mesh(load_library().mpm_mesh_box, _fptr(lo_a), _fptr(hi_a))
 
 
+# ---- scene front-end: OBJ meshes and config.txt ----
+
+def obj_triangles(path) -> np.ndarray:
+    """Triangles (T, 9) of an OBJ file (quads split along the shorter diagonal)."""
+    lib = load_library()
+    n = ctypes.c_int(0)
+    _check(lib.mpm_obj_triangles(str(path).encode(), None, 0, ctypes.byref(n)))
+    out = np.zeros((n.value, 9), dtype=np.float32)
+    _check(lib.mpm_obj_triangles(str(path).encode(), _fptr(out), n.value, ctypes.byref(n)))
+    return out
+
+
+def _points(fn, *args) -> np.ndarray:
+    n = ctypes.c_int64(0)
+    _check(fn(*args, None, 0, ctypes.byref(n)))
+    out = np.zeros((3, n.value), dtype=np.float32)
+    _check(fn(*args, _fptr(out), n.value, ctypes.byref(n)))
+    return out
+
+
+def voxelize_obj(path, resolution: float = 100.0) -> np.ndarray:
+    """VoxelEngine::voxelize: (3, n) centres of the surface cells."""
+    return _points(load_library().mpm_voxelize_obj, str(path).encode(), ctypes.c_float(resolution))
+
+
+def obj_water(path, normalize_size: float, scale: float, displacement, lattice: int = 1) -> np.ndarray:
+    """Simulation::Prepare_object: (3, n) water lattice points of a mesh."""
+    d = _f32(displacement, (3,))
+    return _points(load_library().mpm_obj_water, str(path).encode(), ctypes.c_float(normalize_size),
+                   ctypes.c_float(scale), _fptr(d), int(lattice))
+
+
+def obj_collider(path, normalize_size: float, scale: float, displacement) -> np.ndarray:
+    """Simulation::Prepare_triangles: (T, 9) collider triangles of a mesh."""
+    lib = load_library()
+    d = _f32(displacement, (3,))
+    n = ctypes.c_int(0)
+    args = (str(path).encode(), ctypes.c_float(normalize_size), ctypes.c_float(scale), _fptr(d))
+    _check(lib.mpm_obj_collider(*args, None, 0, ctypes.byref(n)))
+    out = np.zeros((n.value, 9), dtype=np.float32)
+    _check(lib.mpm_obj_collider(*args, _fptr(out), n.value, ctypes.byref(n)))
+    return out
+
+
+def scene_from_config(path):
+    """The reference's Simulation constructor without the GPU: (SimConfig,
+    particles (26, n), triangles (T, 9)) from a config.txt and its models."""
+    lib = load_library()
+    cfg = SimConfig()
+    n = ctypes.c_int64(0)
+    nt = ctypes.c_int(0)
+    p = str(path).encode()
+    _check(lib.mpm_scene_from_config(p, ctypes.byref(cfg), None, 0, ctypes.byref(n), None, 0, ctypes.byref(nt)))
+    parts = np.zeros((PARTICLE_ATTRS, n.value), dtype=np.float32)
+    tris = np.zeros((nt.value, 9), dtype=np.float32)
+    _check(lib.mpm_scene_from_config(p, ctypes.byref(cfg), _fptr(parts), n.value, ctypes.byref(n),
+                                     _fptr(tris), nt.value, ctypes.byref(nt)))
+    return cfg, parts, tris
+
+
 # ---- engine ----
 
 STAGES = ("p2g", "g2p", "g2p2g", "grid_update", "resort", "corrector")
@@ -377,6 +446,11 @@ class Engine:
         if buf.size != PARTICLE_ATTRS * n:
             raise ValueError(f"buffer has {buf.size} floats, expected {PARTICLE_ATTRS * n}")
         _check(self._lib.mpm_engine_init_particles(self._h, n, _fptr(buf)))
+
+    def load_state(self, state) -> None:
+        """Resumes from a getParticles() state (F, v, C kept as they are)."""
+        st = _f32(state).reshape(PARTICLE_ATTRS, -1)
+        _check(self._lib.mpm_engine_load_state(self._h, st.shape[1], _fptr(np.ascontiguousarray(st))))
 
     # Engine::initSDF (engine.cu:105-114)
     def initSDF(self, triangles) -> None:

@@ -33,10 +33,11 @@ HIP_FLAGS = [
     f"-I{ROOT / 'include'}",
     f"-I{CSRC}",
 ]
-CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+# -ffp-contract=off: the scene front-end restates FP32 arithmetic step by step
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 
 HIP_SOURCES = ["kernels_transfer.hip", "kernels_grid.hip", "kernels_sort.hip", "engine.hip", "corrector.hip"]
-CXX_SOURCES = ["config.cpp", "scene.cpp"]
+CXX_SOURCES = ["config.cpp", "scene.cpp", "voxel.cpp"]
 
 
 def _hipcc() -> str:
@@ -75,6 +76,12 @@ def build(verbose: bool = False) -> Path:
     if rc != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{out}")
     os.replace(tmp, LIB)
+    # the headless driver (the reference's program without its window)
+    exe = HERE / "mpm_headless"
+    cmd, rc, out = _compile(["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(CSRC / "headless.cpp"),
+                             f"-L{HERE}", "-lmpm", "-Wl,-rpath,$ORIGIN", "-o", str(exe)])
+    if rc != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{out}")
     return LIB
 
 

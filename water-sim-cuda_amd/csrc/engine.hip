@@ -577,7 +577,7 @@ int create_engine(const mpm_sim_config* cfgs_in, int S, mpm_engine** out) {
 
 // Loads counts[s] particles per scene from a 26 x sum(counts) SoA buffer
 // (scenes consecutive) and lays them out.
-int init_particles_impl(mpm_engine* h, const int* counts, const float* soa26) {
+int init_particles_impl(mpm_engine* h, const int* counts, const float* soa26, bool init_f = true) {
     int64_t total = 0;
     for (int s = 0; s < h->S; ++s) {
         if (counts[s] < 0) return fail(MPM_EINVAL, "negative particle count");
@@ -606,7 +606,7 @@ int init_particles_impl(mpm_engine* h, const int* counts, const float* soa26) {
     float* tmp = nullptr;
     if ((rc = dev_alloc(&tmp, (size_t)MPM_PARTICLE_ATTRS * n))) return rc;
     HIPCHK(hipMemcpyAsync(tmp, soa26, sizeof(float) * MPM_PARTICLE_ATTRS * (size_t)n, hipMemcpyHostToDevice, h->stream));
-    launch_init_F(tmp, n, h->stream);
+    if (init_f) launch_init_F(tmp, n, h->stream);
     if ((rc = ensure_sort_scratch(h, (size_t)n)) || (rc = set_chunk_cap(h, h->chunk_cap)) ||
         (rc = dev_alloc(&h->rounds, (size_t)n)) || (rc = ensure_planes(h, 0, (size_t)n)) ||
         (rc = ensure_planes(h, 1, (size_t)n))) {
@@ -791,6 +791,14 @@ int mpm_engine_init_particles(mpm_engine* h, int n, const float* soa26) {
     if (h->S != 1) return fail(MPM_ESTATE, "engine holds several scenes: use mpm_batch_init_particles");
     if (n < 0) return fail(MPM_EINVAL, "negative particle count");
     return init_particles_impl(h, &n, soa26);
+}
+
+int mpm_engine_load_state(mpm_engine* h, int n, const float* soa26) {
+    int rc = check_engine(h, false);
+    if (rc) return rc;
+    if (h->S != 1) return fail(MPM_ESTATE, "engine holds several scenes: use mpm_batch_init_particles");
+    if (n < 0) return fail(MPM_EINVAL, "negative particle count");
+    return init_particles_impl(h, &n, soa26, false);
 }
 
 int mpm_batch_init_particles(mpm_engine* h, const int* counts, const float* soa26) {
