@@ -33,7 +33,6 @@ struct SceneTable {
     const DevCfg* cfgs;
     const int* pstart;  // nscenes + 1 entries
     int nscenes;
-    int local_bits;     // bits of a scene-local base-cell key; the scene id sits above
 };
 
 // A wave-uniform struct read from memory, moved into scalar registers field by

@@ -62,8 +62,6 @@ struct mpm_engine {
     int S = 1;
     std::vector<int> pstart;  // S + 1: particle ranges (caller order)
     int* d_pstart = nullptr;
-    int local_bits = 0;  // bits of a scene-local base-cell key
-    int key_bits = 0;    // local_bits + scene bits
 
     int device = 0;
     hipStream_t stream = nullptr;
@@ -96,6 +94,7 @@ struct mpm_engine {
     float* sdf = nullptr;
 
     SortScratch sc{};
+    BlockTable blocks{};
     uint32_t* overflow = nullptr;  // = sc.counters + 8: [0] tile-overflow particles, [1] extra claim passes
     uint32_t* pinned = nullptr;
 
@@ -216,9 +215,6 @@ void free_particles(mpm_engine* h) {
     dev_free(h->sc.vals[1]);
     dev_free(h->sc.hist);
     dev_free(h->sc.partial);
-    dev_free(h->sc.heads);
-    dev_free(h->sc.seg_start);
-    dev_free(h->sc.seg_nch);
     h->sc.cap = h->sc.hist_cap = h->sc.partial_cap = 0;
     h->chunk_limit = 0;
     h->chunk_alloc = 0;
@@ -231,12 +227,11 @@ int ensure_sort_scratch(mpm_engine* h, size_t n) {
     SortScratch& sc = h->sc;
     if (sc.cap >= n + 1 && sc.keys[0]) return MPM_OK;
     dev_free(sc.keys[0]); dev_free(sc.keys[1]); dev_free(sc.vals[0]); dev_free(sc.vals[1]);
-    dev_free(sc.hist); dev_free(sc.partial); dev_free(sc.heads); dev_free(sc.seg_start); dev_free(sc.seg_nch);
+    dev_free(sc.hist); dev_free(sc.partial);
     const size_t cap = n + 1;
     int rc;
     if ((rc = dev_alloc(&sc.keys[0], cap)) || (rc = dev_alloc(&sc.keys[1], cap)) || (rc = dev_alloc(&sc.vals[0], cap)) ||
-        (rc = dev_alloc(&sc.vals[1], cap)) || (rc = dev_alloc(&sc.heads, cap)) || (rc = dev_alloc(&sc.seg_start, cap + 1)) ||
-        (rc = dev_alloc(&sc.seg_nch, cap)))
+        (rc = dev_alloc(&sc.vals[1], cap)))
         return rc;
     sc.hist_cap = radix_hist_elems(cap);
     sc.partial_cap = scan_partial_elems(std::max(cap, sc.hist_cap)) + 16;
@@ -294,22 +289,38 @@ SceneTable scene_table(const mpm_engine* h) {
     t.cfgs = h->d_cfgs;
     t.pstart = h->d_pstart;
     t.nscenes = h->S;
-    t.local_bits = h->local_bits;
     return t;
 }
 
-// Sorts sc.keys[0]/vals[0] (n scene/base-cell keys), builds the chunk table
-// and gathers the particles from (src, src_stride, src_orig) into the
-// alternate plane buffer.  Nothing here waits for the device: the chunk count
-// stays in device memory and is bounded by chunk_limit by construction.
+// Re-sorts the particles of (src, src_stride, src_orig) -- block buckets, the
+// chunk table, rounds -- into the alternate plane buffer, moving the planes
+// in `mask`.  Nothing here waits for the device: the chunk count stays in
+// device memory and is bounded by chunk_limit by construction.
 int sort_and_reorder(mpm_engine* h, const float* src, size_t src_stride, const int* src_orig, uint32_t mask) {
-    const int n = h->n;
-    const int r = radix_sort_pairs(h->sc, n, h->key_bits, h->stream);
-    build_chunks(h->sc, r, n, h->chunk_cap, h->local_bits, h->chunks, h->chunk_limit, h->stream);
+    ResortArgs a{};
     const int dst = 1 - h->pcur;
-    launch_reorder(h->chunks, h->sc.counters + 3, h->chunk_est, h->sc.keys[r], h->sc.vals[r], src, src_stride,
-                   src_orig, h->planes[dst], h->plane_cap[dst], h->orig[dst], h->rounds, h->sc.counters + 4, mask,
-                   h->stream);
+    a.src = src;
+    a.src_stride = src_stride;
+    a.src_orig = src_orig;
+    a.dst = h->planes[dst];
+    a.dst_stride = h->plane_cap[dst];
+    a.dst_orig = h->orig[dst];
+    a.n = h->n;
+    a.mask = mask;
+    a.scenes = scene_table(h);
+    a.block_scene = h->d_block_scene;
+    a.blocks = h->blocks;
+    for (int i = 0; i < 2; ++i) {
+        a.keys[i] = h->sc.keys[i];
+        a.vals[i] = h->sc.vals[i];
+    }
+    a.counters = h->sc.counters;
+    a.chunks = h->chunks;
+    a.chunk_cap = h->chunk_cap;
+    a.chunk_limit = h->chunk_limit;
+    a.chunk_est = h->chunk_est;
+    a.rounds = h->rounds;
+    launch_resort(a, h->stream);
     HIPCHK(hipGetLastError());
     h->pcur = dst;
     return MPM_OK;
@@ -320,10 +331,6 @@ constexpr uint32_t MASK_POS_F = 0x7u | (0x1FFu << A_F0);
 
 int resort(mpm_engine* h, bool full) {
     StageTimer t(h, ST_SORT);
-    const float* pl = h->planes[h->pcur];
-    const size_t st = h->plane_cap[h->pcur];
-    launch_keys_internal(pl + A_PX * st, pl + A_PY * st, pl + A_PZ * st, h->n, scene_table(h), h->orig[h->pcur],
-                         h->sc.keys[0], h->sc.vals[0], h->stream);
     return sort_and_reorder(h, h->planes[h->pcur], h->plane_cap[h->pcur], h->orig[h->pcur], full ? MASK_ALL : MASK_POS_F);
 }
 
@@ -490,6 +497,47 @@ int finish_steps(mpm_engine* h) {
     return MPM_OK;
 }
 
+// Numbers the flag blocks of every scene in Morton order of (bx, by, bz),
+// scene after scene, and allocates the re-sort buckets (cleared here; each
+// re-sort leaves them cleared).
+int build_block_table(mpm_engine* h) {
+    auto spread = [](uint64_t v) {  // bit i of v -> bit 3i
+        uint64_t r = 0;
+        for (int i = 0; i < 21; ++i) r |= ((v >> i) & 1ull) << (3 * i);
+        return r;
+    };
+    const int nb = h->nfb;
+    std::vector<uint32_t> rank((size_t)nb), block((size_t)nb);
+    std::vector<std::pair<uint64_t, uint32_t>> order;
+    uint32_t next = 0;
+    for (const DevCfg& d : h->dcs) {
+        order.clear();
+        order.reserve((size_t)d.fbx * d.fby * d.fbz);
+        for (int z = 0; z < d.fbz; ++z)
+            for (int y = 0; y < d.fby; ++y)
+                for (int x = 0; x < d.fbx; ++x)
+                    order.emplace_back((spread((uint64_t)z) << 2) | (spread((uint64_t)y) << 1) | spread((uint64_t)x),
+                                       (uint32_t)(d.foff + x + d.fbx * (y + d.fby * z)));
+        std::sort(order.begin(), order.end());
+        for (const auto& o : order) {
+            rank[o.second] = next;
+            block[next] = o.second;
+            ++next;
+        }
+    }
+    BlockTable& b = h->blocks;
+    b.nb = nb;
+    int rc;
+    if ((rc = dev_alloc(&b.rank, (size_t)nb)) || (rc = dev_alloc(&b.block, (size_t)nb)) ||
+        (rc = dev_alloc(&b.count, (size_t)nb)) || (rc = dev_alloc(&b.scan, (size_t)nb)) ||
+        (rc = dev_alloc(&b.partial, scan_partial_elems((size_t)nb) + 16)))
+        return rc;
+    HIPCHK(hipMemcpy(b.rank, rank.data(), rank.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.block, block.data(), block.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(b.count, 0, (size_t)nb * sizeof(uint32_t)));
+    return MPM_OK;
+}
+
 int create_engine(const mpm_sim_config* cfgs_in, int S, mpm_engine** out) {
     if (!cfgs_in || !out) return fail(MPM_EINVAL, "null argument");
     *out = nullptr;
@@ -501,7 +549,6 @@ int create_engine(const mpm_sim_config* cfgs_in, int S, mpm_engine** out) {
     };
     h->S = S;
     int64_t G = 0, nfb = 0;
-    int maxdim = 1;
     for (int s = 0; s < S; ++s) {
         mpm_sim_config cfg = cfgs_in[s];
         if (cfg.SIZE_X == 0 || cfg.SIZE_Y == 0 || cfg.SIZE_Z == 0)
@@ -523,19 +570,11 @@ int create_engine(const mpm_sim_config* cfgs_in, int S, mpm_engine** out) {
         d.foff = (int)nfb;
         G += gs;
         nfb += (int64_t)d.fbx * d.fby * d.fbz;
-        maxdim = std::max(maxdim, std::max(d.sx, std::max(d.sy, d.sz)));
         h->cfgs.push_back(cfg);
         h->dcs.push_back(d);
     }
-    {
-        int bits = 0, sbits = 0;
-        while ((1 << bits) < maxdim) ++bits;
-        while ((1 << sbits) < S) ++sbits;
-        h->local_bits = 3 * std::max(bits, 1);
-        h->key_bits = h->local_bits + sbits;
-        if (h->key_bits > 32)
-            return cleanup(fail(MPM_ELIMIT, "scene count and grid size need more than 32 key bits"));
-    }
+    // bucket keys hold a block rank above 6 cell bits
+    if (nfb >= (1ll << 26)) return cleanup(fail(MPM_ELIMIT, "scenes hold 2^26 or more 4^3 blocks in all"));
     h->G = (size_t)G;
     h->nfb = (int)nfb;
     if (hipGetDevice(&h->device) != hipSuccess) return cleanup(fail(MPM_EHIP, "no HIP device"));
@@ -563,6 +602,7 @@ int create_engine(const mpm_sim_config* cfgs_in, int S, mpm_engine** out) {
             hipMemcpy(h->d_cfgs, h->dcs.data(), (size_t)S * sizeof(DevCfg), hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(fail(MPM_EHIP, "scene table upload failed"));
     }
+    if ((rc = build_block_table(h))) return cleanup(rc);
     if ((rc = dev_alloc(&h->sc.counters, 64))) return cleanup(rc);
     if (hipMemsetAsync(h->sc.counters, 0, 64 * sizeof(uint32_t), h->stream) != hipSuccess)
         return cleanup(fail(MPM_EHIP, "memset"));
@@ -614,8 +654,6 @@ int init_particles_impl(mpm_engine* h, const int* counts, const float* soa26, bo
         return rc;
     }
     h->chunk_est = h->chunk_limit;
-    launch_keys_internal(tmp, tmp + (size_t)n, tmp + 2 * (size_t)n, n, scene_table(h), nullptr, h->sc.keys[0],
-                         h->sc.vals[0], h->stream);
     h->pcur = 1;  // sort_and_reorder writes into buffer 0
     rc = sort_and_reorder(h, tmp, (size_t)n, nullptr, MASK_ALL);
     hipStreamSynchronize(h->stream);
@@ -716,6 +754,11 @@ int mpm_engine_destroy(mpm_engine* h) {
     dev_free(h->check_buf);
     dev_free(h->d_cfgs);
     dev_free(h->d_block_scene);
+    dev_free(h->blocks.rank);
+    dev_free(h->blocks.block);
+    dev_free(h->blocks.count);
+    dev_free(h->blocks.scan);
+    dev_free(h->blocks.partial);
     dev_free(h->sc.counters);
     h->overflow = nullptr;
     if (h->pinned) hipHostFree(h->pinned);

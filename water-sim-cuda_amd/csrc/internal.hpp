@@ -68,15 +68,49 @@ struct SortScratch {
     uint32_t* vals[2];
     uint32_t* hist;      // 256 * ntiles
     uint32_t* partial;   // scan partials
-    // [0] = or, [1] = and, [2] = segments, [3] = chunks, [4] = rounds of the
-    // last layout; the engine keeps its per-call counters at [8..].
+    // [0] = or, [1] = and, [3] = chunks, [4] = rounds of the last layout,
+    // [6..7] = a 64-bit scan total; the engine keeps its per-call counters at
+    // [8..].
     uint32_t* counters;
-    int* heads;          // n + 1
-    int* seg_start;      // n + 1
-    int* seg_nch;        // n + 1
     size_t cap;          // elements
     size_t hist_cap;     // elements of hist
     size_t partial_cap;
+};
+
+// The flag blocks (4^3 nodes) of all scenes numbered densely, scene by scene
+// and within a scene in Morton order of (bx, by, bz), and the per-block
+// buckets of a re-sort.
+struct BlockTable {
+    uint32_t* rank = nullptr;     // nb: rank of each flag block
+    uint32_t* block = nullptr;    // nb: flag block of each rank
+    uint32_t* count = nullptr;    // nb: bucket sizes (all zero between re-sorts)
+    uint64_t* scan = nullptr;     // nb: exclusive prefix of count | chunks << 32
+    uint64_t* partial = nullptr;  // scan partials
+    int nb = 0;
+};
+
+// One re-sort: the particles of (src, src_stride, src_orig) move, grouped by
+// 4^3 block and laid out in rounds, to (dst, dst_stride, dst_orig).
+struct ResortArgs {
+    const float* src;
+    size_t src_stride;
+    const int* src_orig;  // caller index of each source slot, or null (identity)
+    float* dst;
+    size_t dst_stride;
+    int* dst_orig;
+    int n;
+    uint32_t mask;        // planes that move (bit a = attribute a)
+    SceneTable scenes;
+    const uint16_t* block_scene;
+    BlockTable blocks;
+    uint32_t* keys[2];    // n entries each
+    uint32_t* vals[2];
+    uint32_t* counters;   // SortScratch::counters
+    ChunkDesc* chunks;
+    int chunk_cap;
+    int chunk_limit;      // descriptors allocated (bounds the chunk count by construction)
+    int chunk_est;        // chunk count estimate (sizes the reorder grid)
+    uint8_t* rounds;      // n entries
 };
 
 size_t radix_hist_elems(size_t n);
@@ -86,13 +120,9 @@ size_t scan_partial_elems(size_t n);
 // memset node).
 void launch_set_u32(uint32_t* p, int i, uint32_t v, hipStream_t s);
 
-// Keys of position planes (vals = index): the reference's setKeys keys, or
-// the engine's (scene, base-cell) keys (orig: caller index per slot, or null
-// for the identity).
+// The reference's setKeys keys of position planes (vals = index).
 void launch_keys_reference(const float* px, const float* py, const float* pz, int n, uint32_t* keys, uint32_t* vals,
                            hipStream_t s);
-void launch_keys_internal(const float* px, const float* py, const float* pz, int n, const SceneTable& t,
-                          const int* orig, uint32_t* keys, uint32_t* vals, hipStream_t s);
 // Stable LSD radix sort of (keys, vals) pairs; result in keys[r], vals[r] with
 // r returned (0 or 1).  With key_bits > 0 the keys are known to fit in that
 // many bits and only those digits are sorted; with key_bits = 0 the bits on
@@ -102,19 +132,10 @@ int radix_sort_pairs(SortScratch& sc, int n, int key_bits, hipStream_t s);
 // Exclusive scan of n uint32 values; total written to *total_dev.
 void scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* partial, uint32_t* total_dev,
                         hipStream_t s);
-// Splits the sorted base-cell keys into chunks (one 4^3 Morton block per
-// chunk, at most chunk_cap particles).  Writes up to chunk_limit descriptors; the chunk count
-// (possibly above the limit) lands in sc.counters[3].
-void build_chunks(SortScratch& sc, int r, int n, int chunk_cap, int local_bits, ChunkDesc* chunks, int chunk_limit,
-                  hipStream_t s);
-// Gathers the particles into the round layout of their chunks (one wave per
-// chunk; the chunk count is read on the device), writes the round sizes and
-// each chunk's round count, and adds the round counts into *rounds_total.
-// attr_mask selects the planes moved (bit a = attribute a).
-void launch_reorder(ChunkDesc* ch, const uint32_t* nchunks_dev, int chunk_est, const uint32_t* keys_sorted,
-                    const uint32_t* vals_sorted, const float* src, size_t src_stride, const int* src_orig, float* dst,
-                    size_t dst_stride, int* dst_orig, uint8_t* rounds, uint32_t* rounds_total, uint32_t attr_mask,
-                    hipStream_t s);
+// The engine's re-sort (chunk table, round sizes, chunk round counts, their
+// total in counters[4], chunk count in counters[3]); nothing waits for the
+// device.
+void launch_resort(const ResortArgs& a, hipStream_t s);
 
 // ---- learned corrector (corrector.hip) ----
 constexpr int CORR_FEATURES = 66;

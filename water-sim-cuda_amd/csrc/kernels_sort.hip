@@ -1,16 +1,31 @@
-// kernels_sort.hip -- Morton keys, stable LSD radix sort, scans, chunking and
-// the gather that moves particles into the chunked slot layout.
+// kernels_sort.hip -- the re-sort that lays particles out for the transfer
+// kernels, the reference's Morton-key sort, and scans.
 //
 // The reference sorts (key, iota) with thrust::sort_by_key every substep and
-// only uses the permutation as an index (engine.cu:92-102).  Here the sort's
-// output also drives a physical reorder of the particle planes, so the
-// transfer kernels read and write them coalesced.
+// only uses the permutation as an index (engine.cu:92-102).  The engine's
+// re-sort instead moves the particle planes physically, into chunks (at most
+// CHUNK particles of one 4^3 block of base cells) whose particles sit in
+// rounds of distinct base cells:
+//   1. bucket count: each particle's key = (dense Morton rank of its base
+//      cell's block) << 6 | (cell within the block); runs of equal blocks in
+//      a wave take their ranks in the bucket with one atomic;
+//   2. a scan over the blocks of (particles, chunks);
+//   3. a scatter into block order;
+//   4. the chunk table, one thread per block;
+//   5. one workgroup per chunk: a counting sort by cell in LDS, the round
+//      layout, and the gather of the planes.
+// Particles move little between re-sorts, so consecutive slots mostly share a
+// block and the buckets see about one atomic per wave.  The order of the
+// particles of a block is not fixed (atomics), which changes only the order
+// of floating-point sums downstream.
 //
-// Radix sort: 8-bit digits, 4096-key tiles (4 waves x 16 rounds x 64 lanes).
-// Ranking inside a tile is stable: each wave finds the lanes holding the same
-// digit with eight ballots (a wave64 "match"), the lowest such lane updates
-// the wave's running digit count in LDS, and tiles/waves/rounds/lanes are
-// visited in index order.
+// Radix sort (the reference's key order, for inspection): 8-bit digits,
+// 4096-key tiles (4 waves x 16 rounds x 64 lanes).  Ranking inside a tile is
+// stable: each wave finds the lanes holding the same digit with eight ballots
+// (a wave64 "match"), the lowest such lane updates the wave's running digit
+// count in LDS, and tiles/waves/rounds/lanes are visited in index order.
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace mpm {
@@ -127,22 +142,22 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const uint32_t* __
     }
 }
 
-// ---- exclusive scan of uint32 ----
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& total) {
-    __shared__ uint32_t wsum[SC_THREADS / 64];
+// ---- exclusive scan ----
+template <class T> __device__ __forceinline__ T block_exclusive_scan(T v, T& total) {
+    __shared__ T wsum[SC_THREADS / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t x = v;
+    T x = v;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o);
+        const T y = __shfl_up(x, o);
         if (lane >= o) x += y;
     }
     if (lane == 63) wsum[wave] = x;
     __syncthreads();
-    uint32_t wbase = 0u, tot = 0u;
+    T wbase = 0, tot = 0;
 #pragma unroll
     for (int w = 0; w < SC_THREADS / 64; ++w) {
-        const uint32_t s = wsum[w];
+        const T s = wsum[w];
         if (w < wave) wbase += s;
         tot += s;
     }
@@ -151,32 +166,48 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& t
     return wbase + x - v;
 }
 
-__global__ __launch_bounds__(SC_THREADS) void k_scan_reduce(const uint32_t* __restrict__ in, size_t n,
-                                                            uint32_t* __restrict__ partial) {
+// Scan inputs: plain uint32 values, or a block bucket's (particles, chunks)
+// packed as count | ceil(count / cap) << 32 (no carry: counts total < 2^31).
+struct LoadU32 {
+    const uint32_t* p;
+    __device__ uint32_t operator()(size_t i) const { return p[i]; }
+};
+struct LoadBucket {
+    const uint32_t* count;
+    uint32_t cap;
+    __device__ uint64_t operator()(size_t i) const {
+        const uint32_t c = count[i];
+        return (uint64_t)c | ((uint64_t)((c + cap - 1u) / cap) << 32);
+    }
+};
+
+template <class T, class L>
+__global__ __launch_bounds__(SC_THREADS) void k_scan_reduce(L in, size_t n, T* __restrict__ partial) {
     const size_t base = (size_t)blockIdx.x * SC_TILE + (size_t)threadIdx.x * SC_ITEMS;
-    uint32_t s = 0u;
+    T s = 0;
 #pragma unroll
     for (int i = 0; i < SC_ITEMS; ++i)
-        if (base + i < n) s += in[base + i];
-    uint32_t total;
+        if (base + i < n) s += in(base + i);
+    T total;
     block_exclusive_scan(s, total);
     if (threadIdx.x == 0) partial[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(SC_THREADS) void k_scan_partials(uint32_t* __restrict__ partial, size_t np,
-                                                              uint32_t* __restrict__ total_out) {
-    uint32_t carry = 0u;
+template <class T>
+__global__ __launch_bounds__(SC_THREADS) void k_scan_partials(T* __restrict__ partial, size_t np,
+                                                              T* __restrict__ total_out) {
+    T carry = 0;
     for (size_t off = 0; off < np; off += SC_TILE) {
         const size_t base = off + (size_t)threadIdx.x * SC_ITEMS;
-        uint32_t v[SC_ITEMS];
-        uint32_t s = 0u;
+        T v[SC_ITEMS];
+        T s = 0;
 #pragma unroll
         for (int i = 0; i < SC_ITEMS; ++i) {
-            v[i] = base + i < np ? partial[base + i] : 0u;
+            v[i] = base + i < np ? partial[base + i] : (T)0;
             s += v[i];
         }
-        uint32_t total;
-        uint32_t run = carry + block_exclusive_scan(s, total);
+        T total;
+        T run = carry + block_exclusive_scan(s, total);
 #pragma unroll
         for (int i = 0; i < SC_ITEMS; ++i) {
             if (base + i < np) partial[base + i] = run;
@@ -187,18 +218,20 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_partials(uint32_t* __restri
     if (threadIdx.x == 0 && total_out) *total_out = carry;
 }
 
-__global__ __launch_bounds__(SC_THREADS) void k_scan_apply(const uint32_t* in, uint32_t* out, size_t n,
-                                                           const uint32_t* __restrict__ partial) {
+// `in` may read the array that `out` writes (each thread reads its items
+// before writing them).
+template <class T, class L>
+__global__ __launch_bounds__(SC_THREADS) void k_scan_apply(L in, T* out, size_t n, const T* __restrict__ partial) {
     const size_t base = (size_t)blockIdx.x * SC_TILE + (size_t)threadIdx.x * SC_ITEMS;
-    uint32_t v[SC_ITEMS];
-    uint32_t s = 0u;
+    T v[SC_ITEMS];
+    T s = 0;
 #pragma unroll
     for (int i = 0; i < SC_ITEMS; ++i) {
-        v[i] = base + i < n ? in[base + i] : 0u;
+        v[i] = base + i < n ? in(base + i) : (T)0;
         s += v[i];
     }
-    uint32_t total;
-    uint32_t run = partial[blockIdx.x] + block_exclusive_scan(s, total);
+    T total;
+    T run = partial[blockIdx.x] + block_exclusive_scan(s, total);
 #pragma unroll
     for (int i = 0; i < SC_ITEMS; ++i) {
         if (base + i < n) out[base + i] = run;
@@ -206,10 +239,20 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_apply(const uint32_t* in, u
     }
 }
 
-// ---- keys ----
-// Keys of position planes (vals = index): the engine's base-cell keys
-// (internal) or the reference's setKeys keys.
-// setKeys (kernels.cu:361-371): the reference's Morton keys of the positions.
+template <class T, class L>
+void scan_exclusive(L in, T* out, size_t n, T* partial, T* total_dev, hipStream_t s) {
+    const size_t tiles = (n + SC_TILE - 1) / SC_TILE;
+    if (tiles == 0) return;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_scan_reduce<T, L>), dim3((unsigned)tiles), dim3(SC_THREADS), 0, s, in, n,
+                       partial);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_scan_partials<T>), dim3(1), dim3(SC_THREADS), 0, s, partial, tiles,
+                       total_dev);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_scan_apply<T, L>), dim3((unsigned)tiles), dim3(SC_THREADS), 0, s, in, out, n,
+                       partial);
+}
+
+// ---- the reference's keys ----
+// setKeys (kernels.cu:361-371): Morton keys of the positions, vals = index.
 __global__ void k_keys_reference(const float* __restrict__ px, const float* __restrict__ py,
                                  const float* __restrict__ pz, int n, uint32_t* __restrict__ keys,
                                  uint32_t* __restrict__ vals) {
@@ -219,67 +262,112 @@ __global__ void k_keys_reference(const float* __restrict__ px, const float* __re
     vals[i] = (uint32_t)i;
 }
 
-// The engine's ordering key: scene id above the scene-local Morton code of the
-// base cell.  The scene comes from the caller index (orig, or the slot when
-// orig is null).
-__global__ void k_keys_internal(const float* __restrict__ px, const float* __restrict__ py,
-                                const float* __restrict__ pz, int n, SceneTable t, const int* __restrict__ orig,
-                                uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int sc = t.nscenes > 1 ? scene_of(t, orig ? orig[i] : i) : 0;
-    keys[i] = ((uint32_t)sc << t.local_bits) | base_key(t.cfgs[sc], px[i], py[i], pz[i]);
-    vals[i] = (uint32_t)i;
+// ---- the engine's re-sort ----
+__device__ __forceinline__ uint64_t lanemask_le(int lane) { return (2ull << lane) - 1ull; }
+
+// Bucket key of a position: the rank of its base cell's 4^3 block (base cells
+// clamped into the grid) above the 6-bit Morton index of the cell within the
+// block, so that 8 consecutive cell indices form a 2x2x2 group.
+__device__ __forceinline__ uint32_t bucket_key(const DevCfg& c, const uint32_t* __restrict__ brank, float x, float y,
+                                               float z) {
+    const int bx = min(max(base_of(x), 0), c.sx - 1);
+    const int by = min(max(base_of(y), 0), c.sy - 1);
+    const int bz = min(max(base_of(z), 0), c.sz - 1);
+    const int fb = c.foff + (bx >> 2) + c.fbx * ((by >> 2) + c.fby * (bz >> 2));
+    const uint32_t cell = (uint32_t)((bx & 1) | ((by & 1) << 1) | ((bz & 1) << 2) | ((bx & 2) << 2) |
+                                     ((by & 2) << 3) | ((bz & 2) << 4));
+    return (brank[fb] << 6) | cell;
 }
 
-// ---- chunking ----
-__global__ void k_seg_heads(const uint32_t* __restrict__ keys, int n, uint32_t* __restrict__ heads) {
+// 1. keys[i] and the particle's place within its bucket (rank[i]); bcount
+// holds zeros on entry and the bucket sizes on exit.  The scene of slot i
+// comes from its caller index (orig, or i when orig is null).
+__global__ __launch_bounds__(256) void k_bucket_count(const float* __restrict__ px, const float* __restrict__ py,
+                                                      const float* __restrict__ pz, int n, SceneTable t,
+                                                      const int* __restrict__ orig,
+                                                      const uint32_t* __restrict__ brank,
+                                                      uint32_t* __restrict__ bcount, uint32_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ rank) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    heads[i] = (i == 0 || (keys[i] >> BLOCK_SHIFT) != (keys[i - 1] >> BLOCK_SHIFT)) ? 1u : 0u;
-}
-
-__global__ void k_seg_starts(const uint32_t* __restrict__ heads, const uint32_t* __restrict__ hscan, int n,
-                             const uint32_t* __restrict__ counters, int* __restrict__ seg_start) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) seg_start[counters[2]] = n;
-    if (i >= n) return;
-    if (heads[i]) seg_start[hscan[i]] = i;
-}
-
-__global__ void k_seg_nch(const int* __restrict__ seg_start, const uint32_t* __restrict__ counters, int n, int cap,
-                          uint32_t* __restrict__ nch) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const int nseg = (int)counters[2];
-    if (s >= nseg) {
-        nch[s] = 0u;
-        return;
+    const int lane = threadIdx.x & 63;
+    const bool valid = i < n;  // the valid lanes of a wave are a prefix
+    uint32_t key = 0u, b = 0xFFFFFFFFu;
+    if (valid) {
+        const int sc = t.nscenes > 1 ? scene_of(t, orig ? orig[i] : i) : 0;
+        key = bucket_key(t.cfgs[sc], brank, px[i], py[i], pz[i]);
+        b = key >> 6;
     }
-    const int len = seg_start[s + 1] - seg_start[s];
-    nch[s] = (uint32_t)((len + cap - 1) / cap);
+    const uint32_t prev = __shfl_up(b, 1);
+    const bool head = valid && (lane == 0 || prev != b);
+    const uint64_t heads = __ballot(head);
+    const int nvalid = __popcll(__ballot(valid));
+    uint32_t got = 0u;
+    if (head) {
+        const uint64_t later = heads & ~lanemask_le(lane);
+        const int end = later ? __ffsll((unsigned long long)later) - 1 : nvalid;
+        got = atomicAdd(bcount + b, (uint32_t)(end - lane));
+    }
+    // the head of a valid lane's run: the highest head at or below it
+    const uint64_t mine = heads & lanemask_le(lane);
+    const int hl = valid ? 63 - __clzll((long long)mine) : lane;
+    const uint32_t base = __shfl(got, hl);
+    if (valid) {
+        keys[i] = key;
+        rank[i] = base + (uint32_t)(lane - hl);
+    }
 }
 
-__global__ void k_chunk_desc(const int* __restrict__ seg_start, const uint32_t* __restrict__ counters, int n,
-                             int cap, const uint32_t* __restrict__ first, const uint32_t* __restrict__ keys,
-                             int local_bits, ChunkDesc* __restrict__ chunks, int limit) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n || s >= (int)counters[2]) return;
-    const int st = seg_start[s], len = seg_start[s + 1] - st;
-    const uint32_t kk = keys[st];
-    const int scene = (int)(local_bits >= 32 ? 0u : kk >> local_bits);
-    const uint32_t k = local_bits >= 32 ? kk : (kk & ((1u << local_bits) - 1u));  // Morton code of the base cell
-    const int cx = (int)(compact_bits(k) & ~3u), cy = (int)(compact_bits(k >> 1) & ~3u),
-              cz = (int)(compact_bits(k >> 2) & ~3u);
-    int c = (int)first[s];
-    for (int off = 0; off < len; off += cap, ++c) {
+// 3. Block order: (key, old slot) to bucket start + rank.
+__global__ __launch_bounds__(256) void k_bucket_scatter(const uint32_t* __restrict__ keys,
+                                                        const uint32_t* __restrict__ rank, int n,
+                                                        const uint64_t* __restrict__ bscan,
+                                                        uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t key = keys[i];
+    const uint32_t pos = (uint32_t)bscan[key >> 6] + rank[i];
+    kout[pos] = key;
+    vout[pos] = (uint32_t)i;
+}
+
+// 4. Chunks of every non-empty bucket (pieces of at most cap particles), and
+// the bucket cleared for the next re-sort.  Thread 0 publishes the chunk count
+// (at most `limit`, which bounds it by construction) and clears the round
+// total.
+__global__ __launch_bounds__(256) void k_bucket_chunks(uint32_t* __restrict__ bcount,
+                                                       const uint64_t* __restrict__ bscan,
+                                                       const uint32_t* __restrict__ bblock,
+                                                       const uint16_t* __restrict__ block_scene,
+                                                       const DevCfg* __restrict__ cfgs, int nb, int cap,
+                                                       uint32_t* __restrict__ counters, ChunkDesc* __restrict__ chunks,
+                                                       int limit) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0) {
+        const uint64_t total = *reinterpret_cast<const uint64_t*>(counters + 6);
+        counters[3] = min((uint32_t)(total >> 32), (uint32_t)limit);
+        counters[4] = 0u;
+    }
+    if (b >= nb) return;
+    const uint32_t cnt = bcount[b];
+    if (cnt == 0u) return;
+    bcount[b] = 0u;
+    const uint64_t sc = bscan[b];
+    const int start = (int)(uint32_t)sc;
+    int c = (int)(sc >> 32);
+    const int fb = (int)bblock[b];
+    const int scene = block_scene[fb];
+    const DevCfg& cf = cfgs[scene];
+    const int local = fb - cf.foff;
+    const int bx = local % cf.fbx, byz = local / cf.fbx;
+    const int by = byz % cf.fby, bz = byz / cf.fby;
+    for (int off = 0; off < (int)cnt; off += cap, ++c) {
         if (c >= limit) return;
         ChunkDesc d;
-        d.start = st + off;
-        d.count = min(cap, len - off);
-        d.ox = cx - TILE_MARGIN;
-        d.oy = cy - TILE_MARGIN;
-        d.oz = cz - TILE_MARGIN;
+        d.start = start + off;
+        d.count = min(cap, (int)cnt - off);
+        d.ox = 4 * bx - TILE_MARGIN;
+        d.oy = 4 * by - TILE_MARGIN;
+        d.oz = 4 * bz - TILE_MARGIN;
         d.S = 0;  // set by the reorder
         d.scene = scene;
         d.pad1 = 0;
@@ -287,61 +375,94 @@ __global__ void k_chunk_desc(const int* __restrict__ seg_start, const uint32_t* 
     }
 }
 
-// One wave per chunk: finds the chunk's groups of equal base-cell keys (at
-// most 64: one 4^3 block), then gathers the particles from their old slots
-// (vals) into rounds -- round r takes the r-th member of every group larger
-// than r -- and records the round sizes and the round count.
-__global__ __launch_bounds__(256) void k_reorder(ChunkDesc* __restrict__ ch, const uint32_t* __restrict__ nchunks,
-                                                 const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                                 const float* __restrict__ src, size_t src_stride,
-                                                 const int* __restrict__ src_orig, float* __restrict__ dst,
-                                                 size_t dst_stride, int* __restrict__ dst_orig,
-                                                 uint8_t* __restrict__ rounds, uint32_t* __restrict__ rounds_total,
-                                                 uint32_t mask) {
-    __shared__ int gstart_lds[4][65];
-    const uint32_t nch = *nchunks;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int* gs = gstart_lds[wave];
-    for (uint32_t ci = blockIdx.x * 4u + wave; ci < nch; ci += gridDim.x * 4u) {
-        const ChunkDesc d = ch[ci];
-        int ng = 0;
-        for (int base = 0; base < d.count; base += 64) {
-            const int q = base + lane;
-            const bool head = q < d.count && (q == 0 || keys[d.start + q] != keys[d.start + q - 1]);
-            const uint64_t m = __ballot(head);
-            if (head) gs[ng + __popcll(m & lanemask_lt(lane))] = q;
-            ng += __popcll(m);
+constexpr int RO_THREADS = 256;
+
+// 5. One workgroup per chunk (grid-stride over the chunk table).  In LDS: the
+// chunk's particles counted and ranked by cell (at most 64 cells: one 4^3
+// block), their old slots put in cell order, and the rounds laid out -- round
+// r takes the r-th member of every cell with more than r -- so the lanes of a
+// round hold distinct base cells.  Then the planes in `mask` (bit a =
+// attribute a) and the caller indices move to the new slots.  Dynamic LDS:
+// 3 x cap words.
+__global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ ch,
+                                                        const uint32_t* __restrict__ counters, int cap,
+                                                        const uint32_t* __restrict__ keys,
+                                                        const uint32_t* __restrict__ vals,
+                                                        const float* __restrict__ src, size_t src_stride,
+                                                        const int* __restrict__ src_orig, float* __restrict__ dst,
+                                                        size_t dst_stride, int* __restrict__ dst_orig,
+                                                        uint8_t* __restrict__ rounds,
+                                                        uint32_t* __restrict__ rounds_total, uint32_t mask) {
+    extern __shared__ uint32_t ro_lds[];
+    uint32_t* cell_rank = ro_lds;       // [cap] cell | rank << 6 of each chunk position
+    uint32_t* by_cell = ro_lds + cap;   // [cap] old slots in cell order
+    uint32_t* slot_of = ro_lds + 2 * cap;  // [cap] old slots of the chunk, then of each new slot
+    __shared__ uint32_t cnt[64];
+    __shared__ uint32_t cfirst[64];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const uint32_t nch = counters[3];
+    uint32_t rsum = 0u;
+    for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
+        const ChunkDesc d = uniform(ch[ci]);
+        if (tid < 64) cnt[tid] = 0u;
+        __syncthreads();
+        for (int q = tid; q < d.count; q += RO_THREADS) {
+            const uint32_t c = keys[d.start + q] & 63u;
+            slot_of[q] = vals[d.start + q];
+            cell_rank[q] = c | (atomicAdd(&cnt[c], 1u) << 6);
         }
-        if (lane == 0) gs[ng] = d.count;
-        // The wave's LDS operations execute in order; the clobber keeps the
-        // compiler from moving the reads above the writes.
-        asm volatile("" ::: "memory");
-        const int g0 = lane < ng ? gs[lane] : 0;
-        const int gsize = lane < ng ? gs[lane + 1] - g0 : 0;
-        int S = gsize;
+        __syncthreads();
+        if (tid < 64) {
+            const uint32_t size = cnt[lane];
+            uint32_t x = size;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) S = max(S, __shfl_xor(S, o));
-        int off = 0;
-        for (int r = 0; r < S; ++r) {
-            const bool has = gsize > r;
-            const uint64_t m = __ballot(has);
-            if (has) {
-                const uint32_t si = vals[d.start + g0 + r];
-                const size_t slot = (size_t)d.start + off + __popcll(m & lanemask_lt(lane));
-#pragma unroll 4
-                for (int a = 0; a < A_STATE; ++a)
-                    if (mask & (1u << a)) dst[(size_t)a * dst_stride + slot] = src[(size_t)a * src_stride + si];
-                dst_orig[slot] = src_orig ? src_orig[si] : (int)si;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o);
+                if (lane >= o) x += y;
             }
-            if (lane == 0) rounds[d.start + r] = (uint8_t)__popcll(m);
-            off += __popcll(m);
+            cfirst[lane] = x - size;
         }
-        if (lane == 0) {
-            ch[ci].S = S;
-            atomicAdd(rounds_total, (uint32_t)S);
+        __syncthreads();
+        for (int q = tid; q < d.count; q += RO_THREADS) {
+            const uint32_t v = cell_rank[q];
+            by_cell[cfirst[v & 63u] + (v >> 6)] = slot_of[q];
         }
-        asm volatile("" ::: "memory");  // gs is reused by the wave's next chunk
+        __syncthreads();
+        if (tid < 64) {
+            const uint32_t size = cnt[lane], first = cfirst[lane];
+            uint32_t S = size;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) S = max(S, (uint32_t)__shfl_xor((int)S, o));
+            S = __builtin_amdgcn_readfirstlane(S);
+            uint32_t off = 0u;
+            for (uint32_t r = 0; r < S; ++r) {
+                const bool has = size > r;
+                const uint64_t m = __ballot(has);
+                if (has) slot_of[off + (uint32_t)__popcll(m & lanemask_lt(lane))] = by_cell[first + r];
+                if (lane == 0) rounds[d.start + r] = (uint8_t)__popcll(m);
+                off += (uint32_t)__popcll(m);
+            }
+            if (lane == 0) {
+                ch[ci].S = (int)S;
+                rsum += S;
+            }
+        }
+        __syncthreads();
+        for (int q = tid; q < d.count; q += RO_THREADS) {
+            const uint32_t si = slot_of[q];
+            const size_t o = (size_t)d.start + q;
+            float v[A_STATE];
+#pragma unroll
+            for (int a = 0; a < A_STATE; ++a)
+                if (mask & (1u << a)) v[a] = src[(size_t)a * src_stride + si];
+#pragma unroll
+            for (int a = 0; a < A_STATE; ++a)
+                if (mask & (1u << a)) dst[(size_t)a * dst_stride + o] = v[a];
+            dst_orig[o] = src_orig ? src_orig[si] : (int)si;
+        }
+        __syncthreads();  // the LDS arrays are reused by the next chunk
     }
+    if (tid == 0 && rsum) atomicAdd(rounds_total, rsum);
 }
 
 }  // namespace
@@ -351,14 +472,11 @@ size_t scan_partial_elems(size_t n) { return (n + SC_TILE - 1) / SC_TILE + 1; }
 
 void scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* partial, uint32_t* total_dev,
                         hipStream_t s) {
-    const size_t tiles = (n + SC_TILE - 1) / SC_TILE;
-    if (tiles == 0) {
+    if (n == 0) {
         if (total_dev) hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, total_dev, 0, 0u);
         return;
     }
-    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)tiles), dim3(SC_THREADS), 0, s, in, n, partial);
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(SC_THREADS), 0, s, partial, tiles, total_dev);
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)tiles), dim3(SC_THREADS), 0, s, in, out, n, partial);
+    scan_exclusive<uint32_t>(LoadU32{in}, out, n, partial, total_dev, s);
 }
 
 void launch_set_u32(uint32_t* p, int i, uint32_t v, hipStream_t s) {
@@ -369,12 +487,6 @@ void launch_keys_reference(const float* px, const float* py, const float* pz, in
                            hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_keys_reference, dim3((n + 255) / 256), dim3(256), 0, s, px, py, pz, n, keys, vals);
-}
-
-void launch_keys_internal(const float* px, const float* py, const float* pz, int n, const SceneTable& t,
-                          const int* orig, uint32_t* keys, uint32_t* vals, hipStream_t s) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_keys_internal, dim3((n + 255) / 256), dim3(256), 0, s, px, py, pz, n, t, orig, keys, vals);
 }
 
 int radix_sort_pairs(SortScratch& sc, int n, int key_bits, hipStream_t s) {
@@ -406,31 +518,27 @@ int radix_sort_pairs(SortScratch& sc, int n, int key_bits, hipStream_t s) {
     return r;
 }
 
-void build_chunks(SortScratch& sc, int r, int n, int chunk_cap, int local_bits, ChunkDesc* chunks, int chunk_limit,
-                  hipStream_t s) {
+void launch_resort(const ResortArgs& a, hipStream_t s) {
+    const int n = a.n;
     const unsigned blocks = (unsigned)((n + 255) / 256);
-    uint32_t* heads = reinterpret_cast<uint32_t*>(sc.heads);
-    uint32_t* hscan = sc.keys[1 - r];  // free after the sort
-    uint32_t* nch = reinterpret_cast<uint32_t*>(sc.seg_nch);
-    uint32_t* first = sc.vals[1 - r];
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, sc.counters, 4, 0u);  // round total of the new layout
-    hipLaunchKernelGGL(k_seg_heads, dim3(blocks), dim3(256), 0, s, sc.keys[r], n, heads);
-    scan_exclusive_u32(heads, hscan, (size_t)n, sc.partial, sc.counters + 2, s);
-    hipLaunchKernelGGL(k_seg_starts, dim3(blocks), dim3(256), 0, s, heads, hscan, n, sc.counters, sc.seg_start);
-    hipLaunchKernelGGL(k_seg_nch, dim3(blocks), dim3(256), 0, s, sc.seg_start, sc.counters, n, chunk_cap, nch);
-    scan_exclusive_u32(nch, first, (size_t)n, sc.partial, sc.counters + 3, s);
-    hipLaunchKernelGGL(k_chunk_desc, dim3(blocks), dim3(256), 0, s, sc.seg_start, sc.counters, n, chunk_cap, first,
-                       sc.keys[r], local_bits, chunks, chunk_limit);
-}
-
-void launch_reorder(ChunkDesc* ch, const uint32_t* nchunks_dev, int chunk_est, const uint32_t* keys_sorted,
-                    const uint32_t* vals_sorted, const float* src, size_t src_stride, const int* src_orig, float* dst,
-                    size_t dst_stride, int* dst_orig, uint8_t* rounds, uint32_t* rounds_total, uint32_t attr_mask,
-                    hipStream_t s) {
-    int blocks = (chunk_est + 3) / 4;
-    blocks = blocks < 1 ? 1 : (blocks > 65535 ? 65535 : blocks);
-    hipLaunchKernelGGL(k_reorder, dim3(blocks), dim3(256), 0, s, ch, nchunks_dev, keys_sorted, vals_sorted, src,
-                       src_stride, src_orig, dst, dst_stride, dst_orig, rounds, rounds_total, attr_mask);
+    const BlockTable& bt = a.blocks;
+    if (n > 0)
+        hipLaunchKernelGGL(k_bucket_count, dim3(blocks), dim3(256), 0, s, a.src + (size_t)A_PX * a.src_stride,
+                           a.src + (size_t)A_PY * a.src_stride, a.src + (size_t)A_PZ * a.src_stride, n, a.scenes,
+                           a.src_orig, bt.rank, bt.count, a.keys[0], a.vals[0]);
+    scan_exclusive<uint64_t>(LoadBucket{bt.count, (uint32_t)a.chunk_cap}, bt.scan, (size_t)bt.nb, bt.partial,
+                             reinterpret_cast<uint64_t*>(a.counters + 6), s);
+    if (n > 0)
+        hipLaunchKernelGGL(k_bucket_scatter, dim3(blocks), dim3(256), 0, s, a.keys[0], a.vals[0], n, bt.scan,
+                           a.keys[1], a.vals[1]);
+    hipLaunchKernelGGL(k_bucket_chunks, dim3((unsigned)((bt.nb + 255) / 256)), dim3(256), 0, s, bt.count, bt.scan,
+                       bt.block, a.block_scene, a.scenes.cfgs, bt.nb, a.chunk_cap, a.counters, a.chunks,
+                       a.chunk_limit);
+    if (n <= 0) return;
+    const int grid = std::max(1, std::min(a.chunk_est, 8192));
+    hipLaunchKernelGGL(k_reorder, dim3((unsigned)grid), dim3(RO_THREADS), (size_t)3 * a.chunk_cap * sizeof(uint32_t),
+                       s, a.chunks, a.counters, a.chunk_cap, a.keys[1], a.vals[1], a.src, a.src_stride, a.src_orig,
+                       a.dst, a.dst_stride, a.dst_orig, a.rounds, a.counters + 4, a.mask);
 }
 
 }  // namespace mpm
