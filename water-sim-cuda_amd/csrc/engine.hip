@@ -82,6 +82,10 @@ struct mpm_engine {
     int* orig[2] = {nullptr, nullptr};
     size_t plane_cap[2] = {0, 0};
     int pcur = 0;
+    // A re-sort inside steps() moves only the caller indices; the next G2P
+    // reads positions and F through sc.map from plane set gather_src.
+    bool gather_pending = false;
+    int gather_src = 0;
 
     ChunkDesc* chunks = nullptr;
     uint8_t* rounds = nullptr;  // particles per round of each chunk, at chunk.start + round
@@ -215,6 +219,8 @@ void free_particles(mpm_engine* h) {
     dev_free(h->sc.vals[1]);
     dev_free(h->sc.hist);
     dev_free(h->sc.partial);
+    dev_free(h->sc.map);
+    h->gather_pending = false;
     h->sc.cap = h->sc.hist_cap = h->sc.partial_cap = 0;
     h->chunk_limit = 0;
     h->chunk_alloc = 0;
@@ -227,11 +233,11 @@ int ensure_sort_scratch(mpm_engine* h, size_t n) {
     SortScratch& sc = h->sc;
     if (sc.cap >= n + 1 && sc.keys[0]) return MPM_OK;
     dev_free(sc.keys[0]); dev_free(sc.keys[1]); dev_free(sc.vals[0]); dev_free(sc.vals[1]);
-    dev_free(sc.hist); dev_free(sc.partial);
+    dev_free(sc.hist); dev_free(sc.partial); dev_free(sc.map);
     const size_t cap = n + 1;
     int rc;
     if ((rc = dev_alloc(&sc.keys[0], cap)) || (rc = dev_alloc(&sc.keys[1], cap)) || (rc = dev_alloc(&sc.vals[0], cap)) ||
-        (rc = dev_alloc(&sc.vals[1], cap)))
+        (rc = dev_alloc(&sc.vals[1], cap)) || (rc = dev_alloc(&sc.map, cap)))
         return rc;
     sc.hist_cap = radix_hist_elems(cap);
     sc.partial_cap = scan_partial_elems(std::max(cap, sc.hist_cap)) + 16;
@@ -315,6 +321,7 @@ int sort_and_reorder(mpm_engine* h, const float* src, size_t src_stride, const i
         a.vals[i] = h->sc.vals[i];
     }
     a.counters = h->sc.counters;
+    a.map = h->sc.map;
     a.chunks = h->chunks;
     a.chunk_cap = h->chunk_cap;
     a.chunk_limit = h->chunk_limit;
@@ -327,11 +334,17 @@ int sort_and_reorder(mpm_engine* h, const float* src, size_t src_stride, const i
 }
 
 constexpr uint32_t MASK_ALL = (1u << A_STATE) - 1u;
-constexpr uint32_t MASK_POS_F = 0x7u | (0x1FFu << A_F0);
 
+// full: every plane moves now.  Otherwise (inside steps(), where a G2P always
+// follows) no plane moves: that G2P reads positions and F through the map.
 int resort(mpm_engine* h, bool full) {
     StageTimer t(h, ST_SORT);
-    return sort_and_reorder(h, h->planes[h->pcur], h->plane_cap[h->pcur], h->orig[h->pcur], full ? MASK_ALL : MASK_POS_F);
+    const int src = h->pcur;
+    const int rc = sort_and_reorder(h, h->planes[src], h->plane_cap[src], h->orig[src], full ? MASK_ALL : 0u);
+    if (rc) return rc;
+    h->gather_pending = !full;
+    h->gather_src = src;
+    return MPM_OK;
 }
 
 int transfer_launch_size(const mpm_engine* h) {
@@ -355,13 +368,20 @@ TransferArgs make_args(mpm_engine* h) {
     a.acc_flags = M.flags;
     a.sdf = h->sdf;
     a.overflow = h->overflow;
+    if (h->gather_pending) {
+        a.map = h->sc.map;
+        a.src = h->planes[h->gather_src];
+        a.src_stride = h->plane_cap[h->gather_src];
+    }
     return a;
 }
 
 void run_transfer(mpm_engine* h, int mode, int launch) {
     const int stage = mode == XFER_P2G ? ST_P2G : (mode == XFER_G2P ? ST_G2P : ST_G2P2G);
     StageTimer t(h, stage);
+    // a pending gather is always consumed by a G2P (resort() keeps it so)
     launch_transfer(mode, make_args(h), launch, h->stream);
+    if (mode & XFER_G2P) h->gather_pending = false;
 }
 
 uint32_t* corr_count(mpm_engine* h) { return h->sc.counters + 12; }

@@ -36,6 +36,12 @@ struct TransferArgs {
     uint8_t* acc_flags;
     const float* sdf;
     unsigned int* overflow;  // [0] particles outside their tile, [1] particles that lost a base-node claim
+    // Non-null for the first transfer after a re-sort that moved no planes
+    // (modes with G2P only): slot s reads its position and F from `src` at
+    // old slot map[s]; the kernel writes the particle to `part` as usual.
+    const uint32_t* map;
+    const float* src;
+    size_t src_stride;
 };
 
 // `grid_chunks` sizes the launch (a multiple of 8 workgroups is used); the
@@ -72,6 +78,7 @@ struct SortScratch {
     // [6..7] = a 64-bit scan total; the engine keeps its per-call counters at
     // [8..].
     uint32_t* counters;
+    uint32_t* map;       // old slot of each slot after the last re-sort
     size_t cap;          // elements
     size_t hist_cap;     // elements of hist
     size_t partial_cap;
@@ -89,8 +96,9 @@ struct BlockTable {
     int nb = 0;
 };
 
-// One re-sort: the particles of (src, src_stride, src_orig) move, grouped by
-// 4^3 block and laid out in rounds, to (dst, dst_stride, dst_orig).
+// One re-sort: the particles of (src, src_stride, src_orig) are grouped by
+// 4^3 block and laid out in rounds; map[new slot] = old slot, the caller
+// indices move to dst_orig and the planes in `mask` to (dst, dst_stride).
 struct ResortArgs {
     const float* src;
     size_t src_stride;
@@ -105,6 +113,7 @@ struct ResortArgs {
     BlockTable blocks;
     uint32_t* keys[2];    // n entries each
     uint32_t* vals[2];
+    uint32_t* map;        // n entries: the old slot of each new slot
     uint32_t* counters;   // SortScratch::counters
     ChunkDesc* chunks;
     int chunk_cap;

@@ -381,9 +381,9 @@ constexpr int RO_THREADS = 256;
 // chunk's particles counted and ranked by cell (at most 64 cells: one 4^3
 // block), their old slots put in cell order, and the rounds laid out -- round
 // r takes the r-th member of every cell with more than r -- so the lanes of a
-// round hold distinct base cells.  Then the planes in `mask` (bit a =
-// attribute a) and the caller indices move to the new slots.  Dynamic LDS:
-// 3 x cap words.
+// round hold distinct base cells.  Then map[new slot] = old slot, and the
+// caller indices and the planes in `mask` (bit a = attribute a) move to the
+// new slots.  Dynamic LDS: 3 x cap words.
 __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ ch,
                                                         const uint32_t* __restrict__ counters, int cap,
                                                         const uint32_t* __restrict__ keys,
@@ -392,7 +392,8 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
                                                         const int* __restrict__ src_orig, float* __restrict__ dst,
                                                         size_t dst_stride, int* __restrict__ dst_orig,
                                                         uint8_t* __restrict__ rounds,
-                                                        uint32_t* __restrict__ rounds_total, uint32_t mask) {
+                                                        uint32_t* __restrict__ rounds_total,
+                                                        uint32_t* __restrict__ map, uint32_t mask) {
     extern __shared__ uint32_t ro_lds[];
     uint32_t* cell_rank = ro_lds;       // [cap] cell | rank << 6 of each chunk position
     uint32_t* by_cell = ro_lds + cap;   // [cap] old slots in cell order
@@ -451,6 +452,7 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
         for (int q = tid; q < d.count; q += RO_THREADS) {
             const uint32_t si = slot_of[q];
             const size_t o = (size_t)d.start + q;
+            map[o] = si;
             float v[A_STATE];
 #pragma unroll
             for (int a = 0; a < A_STATE; ++a)
@@ -538,7 +540,7 @@ void launch_resort(const ResortArgs& a, hipStream_t s) {
     const int grid = std::max(1, std::min(a.chunk_est, 8192));
     hipLaunchKernelGGL(k_reorder, dim3((unsigned)grid), dim3(RO_THREADS), (size_t)3 * a.chunk_cap * sizeof(uint32_t),
                        s, a.chunks, a.counters, a.chunk_cap, a.keys[1], a.vals[1], a.src, a.src_stride, a.src_orig,
-                       a.dst, a.dst_stride, a.dst_orig, a.rounds, a.counters + 4, a.mask);
+                       a.dst, a.dst_stride, a.dst_orig, a.rounds, a.counters + 4, a.map, a.mask);
 }
 
 }  // namespace mpm

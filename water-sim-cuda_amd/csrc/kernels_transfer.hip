@@ -272,7 +272,9 @@ __device__ __forceinline__ bool stencil_in_tile(int lx, int ly, int lz) {
            (unsigned)lz <= (unsigned)(TILE_D - 3);
 }
 
-template <int MODE>
+// GATHER: the particles are read through a.map from a.src (the first G2P
+// after a re-sort that left the planes in place).
+template <int MODE, bool GATHER>
 // At most 168 VGPRs: three waves per SIMD (LDS allows about 13 chunks per CU).
 __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a) {
     constexpr bool DO_G2P = (MODE & XFER_G2P) != 0;
@@ -325,7 +327,11 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
     Particle nxt;
     bool nxt_valid = lane < nxt_n;
     uint32_t nxt_slot = (uint32_t)(d.start + lane);
-    if (nxt_valid) load_particle(a.part, a.stride, nxt_slot, nxt, !DO_G2P);
+    auto load = [&](uint32_t slot, Particle& p) {
+        if constexpr (GATHER) load_particle(a.src, a.src_stride, a.map[slot], p, !DO_G2P);
+        else load_particle(a.part, a.stride, slot, p, !DO_G2P);
+    };
+    if (nxt_valid) load(nxt_slot, nxt);
     for (int j = 0; j < S; ++j) {
         const bool valid = nxt_valid;
         const uint32_t slot = nxt_slot;
@@ -334,7 +340,7 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
         nxt_n = j + 1 < S ? (int)a.rounds[d.start + j + 1] : 0;
         nxt_valid = lane < nxt_n;
         nxt_slot = (uint32_t)(d.start + nxt_off + lane);
-        if (nxt_valid) load_particle(a.part, a.stride, nxt_slot, nxt, !DO_G2P);
+        if (nxt_valid) load(nxt_slot, nxt);
         if (valid) {
             if constexpr (DO_G2P) {
                 const int bx = base_of(p.x), by = base_of(p.y), bz = base_of(p.z);
@@ -482,10 +488,17 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
 void launch_transfer(int mode, const TransferArgs& a, int grid_chunks, hipStream_t s) {
     const int per = (grid_chunks + 7) / 8;
     dim3 grid(8 * (per < 1 ? 1 : per)), block(TRANSFER_THREADS);
+    const bool g = a.map != nullptr;  // the engine passes a map only with G2P modes
     switch (mode) {
-        case XFER_P2G: hipLaunchKernelGGL(k_transfer<XFER_P2G>, grid, block, 0, s, a); break;
-        case XFER_G2P: hipLaunchKernelGGL(k_transfer<XFER_G2P>, grid, block, 0, s, a); break;
-        default: hipLaunchKernelGGL(k_transfer<XFER_G2P2G>, grid, block, 0, s, a); break;
+        case XFER_P2G: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_transfer<XFER_P2G, false>), grid, block, 0, s, a); break;
+        case XFER_G2P:
+            if (g) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_transfer<XFER_G2P, true>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_transfer<XFER_G2P, false>), grid, block, 0, s, a);
+            break;
+        default:
+            if (g) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_transfer<XFER_G2P2G, true>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_transfer<XFER_G2P2G, false>), grid, block, 0, s, a);
+            break;
     }
 }
 
