@@ -120,8 +120,13 @@ int mpm_engine_destroy(mpm_engine* h);
  *   "TIMING"        1 = record per-stage HIP events [0]
  *   "GRAPH"         1 = capture a steps(k) call as a HIP graph once and replay
  *                   it while the schedule state matches [0]
+ *   "SDF_BVH"       1 = initSDF through a bounding-volume hierarchy [1]: the
+ *                   same field as the brute-force makeSDF (kernels.cu:282-300,
+ *                   run with 0): equal distances, equal signs wherever the
+ *                   distance is non-zero
  * Read-only (get_option): "GRAPH_REPLAYS" graph launches so far, "SCENES"
- * scene count. */
+ * scene count, "SDF_EXACT_NODES" nodes of the last hierarchy initSDF whose
+ * sign took the exact winding sum. */
 int mpm_engine_set_option(mpm_engine* h, const char* key, int64_t value);
 int mpm_engine_get_option(mpm_engine* h, const char* key, int64_t* value);
 
@@ -132,8 +137,8 @@ int mpm_engine_get_option(mpm_engine* h, const char* key, int64_t* value);
  * array, scene after scene (get_particles/get_positions return it in that
  * order); grids and SDFs are concatenated scene after scene
  * (mpm_batch_scene_info gives the offsets).  mpm_engine_set_sdf/get_sdf and
- * mpm_engine_get_grid address the concatenation.  Limits: nscenes <= 4096 and
- * 3*ceil(log2(largest edge)) + ceil(log2(nscenes)) <= 32. */
+ * mpm_engine_get_grid address the concatenation.  Limits: nscenes <= 4096,
+ * fewer than 2^31 grid nodes and 2^26 4^3-node blocks in all. */
 int mpm_batch_create(const mpm_sim_config* cfgs, int nscenes, mpm_engine** out);
 /* counts[s] particles per scene from a 26 x sum(counts) SoA buffer. */
 int mpm_batch_init_particles(mpm_engine* h, const int* counts, const float* soa26);

@@ -74,6 +74,8 @@ struct mpm_engine {
     bool timing = false;
     bool use_graph = false;
     bool capturing = false;
+    bool sdf_bvh = true;             // makeSDF through the hierarchy (same field)
+    int64_t sdf_exact_nodes = 0;     // nodes of the last makeSDF that took the exact sign sum
 
     size_t G = 0;  // nodes of all scenes
     int nfb = 0;   // flag blocks of all scenes
@@ -699,8 +701,17 @@ int sdf_from_triangles(mpm_engine* h, int scene, const mpm_triangle* tris, int n
     int rc;
     if ((rc = dev_alloc(&dt, (size_t)9 * nt))) return rc;
     HIPCHK(hipMemcpyAsync(dt, tris, sizeof(mpm_triangle) * (size_t)nt, hipMemcpyHostToDevice, h->stream));
-    launch_make_sdf(d, dt, nt, h->sdf, h->stream);
-    hipError_t e = hipStreamSynchronize(h->stream);
+    hipError_t e = hipSuccess;
+    if (h->sdf_bvh) {
+        unsigned int ex = 0;
+        if (launch_make_sdf_bvh(d, dt, reinterpret_cast<const float*>(tris), nt, h->sdf, &ex, h->stream) != 0)
+            e = hipErrorUnknown;
+        h->sdf_exact_nodes = ex;
+    } else {
+        launch_make_sdf(d, dt, nt, h->sdf, h->stream);
+        h->sdf_exact_nodes = 0;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     dev_free(dt);
     if (e != hipSuccess) return fail(MPM_EHIP, std::string("makeSDF: ") + hipGetErrorString(e));
     HIPCHK(hipGetLastError());
@@ -818,6 +829,8 @@ int mpm_engine_set_option(mpm_engine* h, const char* key, int64_t value) {
     } else if (k == "LAUNCH_PCT") {
         if (value < 1 || value > 100) return fail(MPM_EINVAL, "LAUNCH_PCT must be in [1, 100]");
         h->launch_pct = (int)value;
+    } else if (k == "SDF_BVH") {
+        h->sdf_bvh = value != 0;
     } else if (k == "TIMING") {
         h->timing = value != 0;
     } else if (k == "GRAPH") {
@@ -840,6 +853,8 @@ int mpm_engine_get_option(mpm_engine* h, const char* key, int64_t* value) {
     else if (k == "CHUNK") *value = h->chunk_cap;
     else if (k == "LAUNCH_PCT") *value = h->launch_pct;
     else if (k == "TIMING") *value = h->timing ? 1 : 0;
+    else if (k == "SDF_BVH") *value = h->sdf_bvh ? 1 : 0;
+    else if (k == "SDF_EXACT_NODES") *value = h->sdf_exact_nodes;
     else if (k == "GRAPH") *value = h->use_graph ? 1 : 0;
     else if (k == "CORRECT") *value = h->correct ? 1 : 0;
     else if (k == "GRAPH_REPLAYS") *value = h->graph_replays;

@@ -56,6 +56,12 @@ void launch_grid_update(const DevCfg* cfgs, const uint16_t* block_scene, GridBuf
 
 // makeSDF of one scene (c: host copy of its DevCfg) into its part of `sdf`.
 void launch_make_sdf(const DevCfg& c, const float* tris, int nt, float* sdf, hipStream_t s);
+// The same field through a bounding-volume hierarchy built on the host from
+// tris_host (tris_dev: the same triangles on the device, for the exact sign
+// fallback).  Synchronises the stream; *exact_count receives the number of
+// nodes whose sign needed the exact winding sum.  Returns 0 or -1 (HIP error).
+int launch_make_sdf_bvh(const DevCfg& c, const float* tris_dev, const float* tris_host, int nt, float* sdf,
+                        unsigned int* exact_count, hipStream_t s);
 void launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
 void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s);
 void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t s);

@@ -1,6 +1,10 @@
 // kernels_grid.hip -- grid update, mesh SDF, state export and small fills.
 #include <algorithm>
 
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "internal.hpp"
 
 namespace mpm {
@@ -141,6 +145,99 @@ __global__ __launch_bounds__(256) void k_make_sdf(DevCfg c, const float* __restr
     sdf[c.goff + idx] = (wn > 0.5f ? -1.0f : 1.0f) * sqrtf(mind);
 }
 
+// ---- makeSDF over a bounding-volume hierarchy (same results, less work) ----
+// Nodes in depth-first order: an internal node's left child follows it, its
+// right child starts at the left child's `skip`, and `skip` is the first node
+// after its subtree, so both traversals below are stackless loops.
+struct SdfNode {
+    float lo[3];
+    int skip;
+    float hi[3];
+    int first;   // leaf: first triangle of the reordered array
+    float an[3]; // sum over the node's triangles of area x unit normal (= cross / 2)
+    float rad;   // radius of a ball around `cen` holding every vertex
+    float cen[3];
+    int count;   // leaf: triangle count; 0 for internal nodes
+};
+
+// Far-field rule of the winding sum: a node whose ball is seen from farther
+// than SDF_BETA radii contributes its dipole term an . (cen - q) / |cen - q|^3.
+// A node whose approximate winding number lies within SDF_TAU of 1/2 gets the
+// exact sum over all triangles in input order, as makeSDF computes it.
+constexpr float SDF_BETA = 3.0f;
+constexpr float SDF_TAU = 0.25f;
+
+__device__ __forceinline__ float box_dist2(const SdfNode& n, float px, float py, float pz) {
+    const float dx = fmaxf(fmaxf(n.lo[0] - px, px - n.hi[0]), 0.0f);
+    const float dy = fmaxf(fmaxf(n.lo[1] - py, py - n.hi[1]), 0.0f);
+    const float dz = fmaxf(fmaxf(n.lo[2] - pz, pz - n.hi[2]), 0.0f);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// One thread per node.  The distance is the minimum of point_tri_dist2 over
+// every triangle that a box test cannot exclude (with a relative margin that
+// covers rounding), hence the brute force's minimum bit for bit; the sign
+// follows the exact winding sum wherever the approximation is not clearly
+// away from 1/2.
+__global__ __launch_bounds__(256) void k_make_sdf_bvh(DevCfg c, const SdfNode* __restrict__ nodes, int nn,
+                                                      const float* __restrict__ ptris,
+                                                      const float* __restrict__ tris, int nt,
+                                                      float* __restrict__ sdf, unsigned int* __restrict__ exact) {
+    const int G = c.sxy * c.sz;
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= G) return;
+    const int x = idx % c.sx, y = (idx / c.sx) % c.sy, z = idx / c.sxy;
+    const float px = (float)x, py = (float)y, pz = (float)z;
+    // an upper bound from the leaf reached by always taking the nearer box
+    float best = 1e30f;
+    int i = 0;
+    while (nodes[i].count == 0) {
+        const int l = i + 1, r = nodes[l].skip;
+        i = box_dist2(nodes[l], px, py, pz) <= box_dist2(nodes[r], px, py, pz) ? l : r;
+    }
+    for (int t = 0; t < nodes[i].count; ++t)
+        best = fminf(best, point_tri_dist2(px, py, pz, ptris + 9 * (size_t)(nodes[i].first + t)));
+    i = 0;
+    while (i < nn) {
+        const SdfNode& n = nodes[i];
+        if (box_dist2(n, px, py, pz) > best * 1.0001f + 1e-6f) {
+            i = n.skip;
+            continue;
+        }
+        if (n.count) {
+            for (int t = 0; t < n.count; ++t)
+                best = fminf(best, point_tri_dist2(px, py, pz, ptris + 9 * (size_t)(n.first + t)));
+            i = n.skip;
+        } else {
+            ++i;
+        }
+    }
+    float w = 0.0f;
+    i = 0;
+    while (i < nn) {
+        const SdfNode& n = nodes[i];
+        const float dx = n.cen[0] - px, dy = n.cen[1] - py, dz = n.cen[2] - pz;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 > (SDF_BETA * n.rad) * (SDF_BETA * n.rad)) {
+            w += (n.an[0] * dx + n.an[1] * dy + n.an[2] * dz) / (d2 * sqrtf(d2));
+            i = n.skip;
+        } else if (n.count) {
+            for (int t = 0; t < n.count; ++t) w += solid_angle(px, py, pz, ptris + 9 * (size_t)(n.first + t));
+            i = n.skip;
+        } else {
+            ++i;
+        }
+    }
+    float wn = w / (4.0f * 3.1415f);
+    if (fabsf(wn - 0.5f) < SDF_TAU) {
+        float we = 0.0f;
+        for (int t = 0; t < nt; ++t) we += solid_angle(px, py, pz, tris + 9 * (size_t)t);
+        wn = we / (4.0f * 3.1415f);
+        atomicAdd(exact, 1u);
+    }
+    sdf[c.goff + idx] = (wn > 0.5f ? -1.0f : 1.0f) * sqrtf(best);
+}
+
 __global__ void k_fill_f32(float* p, size_t n, float v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -205,6 +302,143 @@ void launch_grid_update(const DevCfg* cfgs, const uint16_t* block_scene, GridBuf
 void launch_make_sdf(const DevCfg& c, const float* tris, int nt, float* sdf, hipStream_t s) {
     const int G = c.sxy * c.sz;
     hipLaunchKernelGGL(k_make_sdf, dim3((G + 255) / 256), dim3(256), 0, s, c, tris, nt, sdf);
+}
+
+namespace {
+
+// Median-split hierarchy over the triangles (host).  perm receives the
+// triangle order of the leaves.
+struct SdfBuilder {
+    const float* tris;
+    std::vector<int> perm;
+    std::vector<SdfNode> nodes;
+    std::vector<float> cx;  // centroids, 3 per triangle
+
+    int build(int begin, int end) {
+        const int id = (int)nodes.size();
+        nodes.push_back(SdfNode{});
+        SdfNode n{};
+        for (int a = 0; a < 3; ++a) {
+            n.lo[a] = 3.4e38f;
+            n.hi[a] = -3.4e38f;
+        }
+        double an[3] = {0, 0, 0}, ac[3] = {0, 0, 0}, area = 0.0, mc[3] = {0, 0, 0};
+        float clo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, chi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+        for (int k = begin; k < end; ++k) {
+            const float* t = tris + 9 * (size_t)perm[k];
+            for (int v = 0; v < 3; ++v)
+                for (int a = 0; a < 3; ++a) {
+                    n.lo[a] = std::min(n.lo[a], t[3 * v + a]);
+                    n.hi[a] = std::max(n.hi[a], t[3 * v + a]);
+                }
+            const double e1[3] = {(double)t[3] - t[0], (double)t[4] - t[1], (double)t[5] - t[2]};
+            const double e2[3] = {(double)t[6] - t[0], (double)t[7] - t[1], (double)t[8] - t[2]};
+            const double cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                                  e1[0] * e2[1] - e1[1] * e2[0]};
+            const double A = 0.5 * std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+            for (int a = 0; a < 3; ++a) {
+                an[a] += 0.5 * cr[a];
+                const float cc = cx[3 * (size_t)perm[k] + a];
+                ac[a] += A * cc;
+                mc[a] += cc;
+                clo[a] = std::min(clo[a], cc);
+                chi[a] = std::max(chi[a], cc);
+            }
+            area += A;
+        }
+        const int cnt = end - begin;
+        for (int a = 0; a < 3; ++a) {
+            n.an[a] = (float)an[a];
+            n.cen[a] = (float)(area > 0.0 ? ac[a] / area : mc[a] / cnt);
+        }
+        double r2 = 0.0;
+        for (int k = begin; k < end; ++k) {
+            const float* t = tris + 9 * (size_t)perm[k];
+            for (int v = 0; v < 3; ++v) {
+                double d2 = 0.0;
+                for (int a = 0; a < 3; ++a) {
+                    const double d = (double)t[3 * v + a] - n.cen[a];
+                    d2 += d * d;
+                }
+                r2 = std::max(r2, d2);
+            }
+        }
+        // rounded up so that the float ball still holds every vertex
+        n.rad = (float)(std::sqrt(r2) * (1.0 + 1e-6) + 1e-6);
+        if (cnt <= 4) {
+            n.first = begin;
+            n.count = cnt;
+            n.skip = id + 1;
+            nodes[id] = n;
+            return id;
+        }
+        int axis = 0;
+        for (int a = 1; a < 3; ++a)
+            if (chi[a] - clo[a] > chi[axis] - clo[axis]) axis = a;
+        const int mid = begin + cnt / 2;
+        std::nth_element(perm.begin() + begin, perm.begin() + mid, perm.begin() + end, [&](int p, int q) {
+            const float a = cx[3 * (size_t)p + axis], b = cx[3 * (size_t)q + axis];
+            return a < b || (a == b && p < q);
+        });
+        n.first = -1;
+        n.count = 0;
+        nodes[id] = n;
+        build(begin, mid);
+        build(mid, end);
+        nodes[id].skip = (int)nodes.size();
+        return id;
+    }
+};
+
+}  // namespace
+
+int launch_make_sdf_bvh(const DevCfg& c, const float* tris_dev, const float* tris_host, int nt, float* sdf,
+                        unsigned int* exact_count, hipStream_t s) {
+    SdfBuilder b;
+    b.tris = tris_host;
+    b.perm.resize((size_t)nt);
+    b.cx.resize((size_t)3 * nt);
+    for (int t = 0; t < nt; ++t) {
+        b.perm[t] = t;
+        for (int a = 0; a < 3; ++a)
+            b.cx[3 * (size_t)t + a] = (tris_host[9 * (size_t)t + a] + tris_host[9 * (size_t)t + 3 + a] +
+                                       tris_host[9 * (size_t)t + 6 + a]) / 3.0f;
+    }
+    b.nodes.reserve((size_t)2 * nt);
+    b.build(0, nt);
+    std::vector<float> pt((size_t)9 * nt);
+    for (int k = 0; k < nt; ++k)
+        std::copy(tris_host + 9 * (size_t)b.perm[k], tris_host + 9 * (size_t)b.perm[k] + 9, pt.begin() + 9 * (size_t)k);
+    SdfNode* dn = nullptr;
+    float* dp = nullptr;
+    unsigned int* dc = nullptr;
+    if (hipMalloc((void**)&dn, b.nodes.size() * sizeof(SdfNode)) != hipSuccess) return -1;
+    if (hipMalloc((void**)&dp, pt.size() * sizeof(float)) != hipSuccess) {
+        hipFree(dn);
+        return -1;
+    }
+    if (hipMalloc((void**)&dc, sizeof(unsigned int)) != hipSuccess) {
+        hipFree(dn);
+        hipFree(dp);
+        return -1;
+    }
+    hipError_t e = hipMemcpyAsync(dn, b.nodes.data(), b.nodes.size() * sizeof(SdfNode), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dp, pt.data(), pt.size() * sizeof(float), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemsetAsync(dc, 0, sizeof(unsigned int), s);
+    const int G = c.sxy * c.sz;
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_make_sdf_bvh, dim3((G + 255) / 256), dim3(256), 0, s, c, dn, (int)b.nodes.size(), dp,
+                           tris_dev, nt, sdf, dc);
+        e = hipGetLastError();
+    }
+    unsigned int cnt = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&cnt, dc, sizeof(unsigned int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(dn);
+    hipFree(dp);
+    hipFree(dc);
+    if (exact_count) *exact_count = cnt;
+    return e == hipSuccess ? 0 : -1;
 }
 
 void launch_fill_f32(float* p, size_t n, float v, hipStream_t s) {
