@@ -154,7 +154,8 @@ int mpm_batch_scene_info(mpm_engine* h, int scene, int64_t* out4);
 /* Engine::init + initParticles + initGrid (engine.cu:45-52, :72-90) and
  * initFMatrices (kernels.cu:349-358): copies n particles from the 26*n host
  * buffer, sets F's diagonal to 1, V = RESOLUTION^3, m = 1, zeroes the grid and
- * performs the initial Morton sort.  n = 0 is allowed (steps are no-ops). */
+ * lays the particles out for the transfer kernels (the initial sort).  n = 0 is
+ * allowed (steps are no-ops). */
 int mpm_engine_init_particles(mpm_engine* h, int n, const float* soa26);
 /* Loads a saved 26*n state (get_particles output) as it is: like
  * init_particles but without initFMatrices, so F, v and C continue (no
