@@ -96,6 +96,7 @@ struct mpm_engine {
     int chunk_est = 0;    // chunk count at the last host synchronisation (sizes launches)
 
     GridBuf grid[2] = {};
+    uint2* grid_list = nullptr;  // nfb: flagged row segments of a grid update
     int vcur = 0;  // grid[vcur]: velocities of the last update; grid[1 - vcur]: accumulates
     float* sdf = nullptr;
 
@@ -391,8 +392,8 @@ uint32_t* corr_count(mpm_engine* h) { return h->sc.counters + 12; }
 void run_grid_update(mpm_engine* h) {
     {
         StageTimer t(h, ST_GRID);
-        launch_grid_update(h->d_cfgs, h->d_block_scene, h->grid[1 - h->vcur], h->grid[h->vcur], true, h->sdf, h->nfb,
-                           h->stream);
+        launch_grid_update(h->d_cfgs, h->d_block_scene, h->S, h->grid[1 - h->vcur], h->grid[h->vcur], true, h->sdf,
+                           h->nfb, h->grid_list, h->sc.counters + 14, h->stream);
         h->vcur = 1 - h->vcur;
     }
     const GridBuf& V = h->grid[h->vcur];
@@ -613,7 +614,8 @@ int create_engine(const mpm_sim_config* cfgs_in, int S, mpm_engine** out) {
     if ((rc = dev_alloc(&h->sdf, h->G))) return cleanup(rc);
     launch_fill_f32(h->sdf, h->G, 1e15f, h->stream);  // makeSDF with no triangles
     if ((rc = dev_alloc(&h->d_cfgs, (size_t)S))) return cleanup(rc);
-    if ((rc = dev_alloc(&h->d_block_scene, (size_t)h->nfb))) return cleanup(rc);
+    if ((rc = dev_alloc(&h->d_block_scene, (size_t)h->nfb)) || (rc = dev_alloc(&h->grid_list, (size_t)h->nfb)))
+        return cleanup(rc);
     {
         std::vector<uint16_t> bs((size_t)h->nfb);
         for (int s = 0; s < S; ++s) {
@@ -785,6 +787,7 @@ int mpm_engine_destroy(mpm_engine* h) {
     dev_free(h->check_buf);
     dev_free(h->d_cfgs);
     dev_free(h->d_block_scene);
+    dev_free(h->grid_list);
     dev_free(h->blocks.rank);
     dev_free(h->blocks.block);
     dev_free(h->blocks.count);
@@ -1267,7 +1270,8 @@ int mpm_engine_debug_grid_update(mpm_engine* h, const float* in, float* out) {
     for (int p = 0; p < 4; ++p)
         HIPCHK(hipMemcpyAsync(h->grid[m].plane[p], in + p * h->G, h->G * sizeof(float), hipMemcpyHostToDevice, h->stream));
     launch_fill_u8(h->grid[m].flags, (size_t)h->nfb, 1, h->stream);
-    launch_grid_update(h->d_cfgs, h->d_block_scene, h->grid[m], h->grid[h->vcur], false, h->sdf, h->nfb, h->stream);
+    launch_grid_update(h->d_cfgs, h->d_block_scene, h->S, h->grid[m], h->grid[h->vcur], false, h->sdf, h->nfb,
+                       h->grid_list, h->sc.counters + 14, h->stream);
     if ((rc = copy_grid_out(h, m, 0, h->G, out, h->G))) return rc;
     return zero_grid(h, m);
 }

@@ -50,9 +50,10 @@ void launch_transfer(int mode, const TransferArgs& a, int grid_chunks, hipStream
 
 // Grid update over the 4^3 blocks flagged in `m` (blocks of all scenes,
 // block_scene[b] = scene of block b), zeroing the blocks flagged in `v_clear`
-// (the previous velocity buffer) and clearing those flags.
-void launch_grid_update(const DevCfg* cfgs, const uint16_t* block_scene, GridBuf m, GridBuf v_clear, bool clear_v,
-                        const float* sdf, int nblocks, hipStream_t s);
+// (the previous velocity buffer) and clearing those flags.  list (nblocks
+// entries) and *count are scratch.
+void launch_grid_update(const DevCfg* cfgs, const uint16_t* block_scene, int nscenes, GridBuf m, GridBuf v_clear,
+                        bool clear_v, const float* sdf, int nblocks, uint2* list, uint32_t* count, hipStream_t s);
 
 // makeSDF of one scene (c: host copy of its DevCfg) into its part of `sdf`.
 void launch_make_sdf(const DevCfg& c, const float* tris, int nt, float* sdf, hipStream_t s);

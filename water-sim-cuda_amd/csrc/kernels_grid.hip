@@ -1,7 +1,5 @@
 // kernels_grid.hip -- grid update, mesh SDF, state export and small fills.
 #include <algorithm>
-
-#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -10,62 +8,150 @@
 namespace mpm {
 namespace {
 
-// gridUpdate, kernels.cu:144-196, restricted to the 4^3 node blocks that the
-// last P2G touched.  One wave per block (lane = node, x fastest).  The same
-// launch zeroes the blocks of the previous velocity buffer that its own P2G
-// had touched, so that buffer is all-zero again when it next accumulates.
-// Blocks of all scenes are numbered consecutively; block_scene maps a block to
-// its scene.
-__global__ __launch_bounds__(256) void k_grid_update(const DevCfg* __restrict__ cfgs,
-                                                     const uint16_t* __restrict__ block_scene, GridBuf m, GridBuf v,
-                                                     int clear_v, const float* __restrict__ sdf, int nblocks) {
-    const int blk = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-    if (blk >= nblocks) return;
-    const int lane = threadIdx.x & 63;
-    const bool fm = m.flags[blk] != 0;
-    const bool fv = clear_v && v.flags[blk] != 0;
-    if (!fm && !fv) return;
-    const DevCfg c = uniform(cfgs[block_scene[blk]]);
-    const int lb = blk - c.foff;
-    const int bx = lb % c.fbx, by = (lb / c.fbx) % c.fby, bz = lb / (c.fbx * c.fby);
-    const int x = bx * 4 + (lane & 3), y = by * 4 + ((lane >> 2) & 3), z = bz * 4 + (lane >> 4);
-    if (inb(c, x, y, z)) {
-        const int g = gidx(c, x, y, z);
-        if (fv) {
-            v.plane[0][g] = 0.0f;
-            v.plane[1][g] = 0.0f;
-            v.plane[2][g] = 0.0f;
-            v.plane[3][g] = 0.0f;
-        }
-        if (fm) {
-            const float mass = m.plane[0][g];
-            float vx = 0.0f, vy = 0.0f, vz = 0.0f;
-            if (mass > 1e-9f) {
-                const float dist = sdf[g];
-                vx = m.plane[1][g] / mass;
-                vy = m.plane[2][g] / mass - c.gravity * c.dt;
-                vz = m.plane[3][g] / mass;
-                if (dist < 0.0f) {
-                    float nx, ny, nz;
-                    node_normal(c, sdf, x, y, z, nx, ny, nz);
-                    float vn = vx * nx + vy * ny + vz * nz;
-                    if (vn < 0.0f) {
-                        vx -= nx * vn; vy -= ny * vn; vz -= nz * vn;
-                    }
-                }
-                if ((uint32_t)x < c.pad && vx < 0) vx = 0.0f;
-                if ((uint32_t)y < c.pad && vy < 0) vy = 0.0f;
-                if ((uint32_t)z < c.pad && vz < 0) vz = 0.0f;
-                if ((uint32_t)x > (uint32_t)c.sx - 1u - c.pad && vx > 0) vx = 0.0f;
-                if ((uint32_t)y > (uint32_t)c.sy - 1u - c.pad && vy > 0) vy = 0.0f;
-                if ((uint32_t)z > (uint32_t)c.sz - 1u - c.pad && vz > 0) vz = 0.0f;
+// gridUpdate, kernels.cu:144-196, per node: velocity from momentum, gravity,
+// collider slip, PADDING walls.  Split into the node's loads and the rest, so
+// that a wave can have several nodes' loads in flight.
+struct GuNode {
+    int g;  // < 0: nothing to do
+    float mass, mx, my, mz, dist;
+};
+
+__device__ __forceinline__ GuNode grid_update_load(GridBuf m, const float* __restrict__ sdf, int g, bool fm) {
+    GuNode n;
+    n.g = g;
+    n.mass = n.mx = n.my = n.mz = n.dist = 0.0f;
+    if (fm && g >= 0) {
+        n.mass = m.plane[0][g];
+        n.mx = m.plane[1][g];
+        n.my = m.plane[2][g];
+        n.mz = m.plane[3][g];
+        n.dist = sdf[g];
+    }
+    return n;
+}
+
+// fm: the node's block was touched by the last P2G (update it); fv: the
+// node's block of the previous velocity buffer is to be zeroed.
+__device__ __forceinline__ void grid_update_store(const DevCfg& c, GridBuf m, GridBuf v, const float* __restrict__ sdf,
+                                                  const GuNode& n, int x, int y, int z, bool fm, bool fv) {
+    if (n.g < 0) return;
+    const int g = n.g;
+    if (fv) {
+        v.plane[0][g] = 0.0f;
+        v.plane[1][g] = 0.0f;
+        v.plane[2][g] = 0.0f;
+        v.plane[3][g] = 0.0f;
+    }
+    if (!fm) return;
+    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    if (n.mass > 1e-9f) {
+        vx = n.mx / n.mass;
+        vy = n.my / n.mass - c.gravity * c.dt;
+        vz = n.mz / n.mass;
+        if (n.dist < 0.0f) {
+            float nx, ny, nz;
+            node_normal(c, sdf, x, y, z, nx, ny, nz);
+            float vn = vx * nx + vy * ny + vz * nz;
+            if (vn < 0.0f) {
+                vx -= nx * vn; vy -= ny * vn; vz -= nz * vn;
             }
-            m.plane[1][g] = vx;
-            m.plane[2][g] = vy;
-            m.plane[3][g] = vz;
+        }
+        if ((uint32_t)x < c.pad && vx < 0) vx = 0.0f;
+        if ((uint32_t)y < c.pad && vy < 0) vy = 0.0f;
+        if ((uint32_t)z < c.pad && vz < 0) vz = 0.0f;
+        if ((uint32_t)x > (uint32_t)c.sx - 1u - c.pad && vx > 0) vx = 0.0f;
+        if ((uint32_t)y > (uint32_t)c.sy - 1u - c.pad && vy > 0) vy = 0.0f;
+        if ((uint32_t)z > (uint32_t)c.sz - 1u - c.pad && vz > 0) vz = 0.0f;
+    }
+    m.plane[1][g] = vx;
+    m.plane[2][g] = vy;
+    m.plane[3][g] = vz;
+}
+
+// The grid update is restricted to the 4^3 node blocks that the last P2G
+// touched; the same pass zeroes the blocks of the previous velocity buffer
+// that its own P2G had touched (and clears their flags), so that buffer is
+// all-zero again when it next accumulates.  Work goes by row segments of four
+// x-adjacent blocks (16 x 4 x 4 nodes), so that a wave's accesses cover 64
+// contiguous bytes per grid row.
+//
+// 1. The segments with a flagged block: entry {first block, touched mask |
+// to-clear mask << 4} (bit k: block first + k).  One thread per block; those
+// with bx % 4 == 0 read their segment's flags.  One list append (atomic) per
+// workgroup.  *count is zero on entry.
+__global__ __launch_bounds__(256) void k_grid_list(const DevCfg* __restrict__ cfgs,
+                                                   const uint16_t* __restrict__ block_scene,
+                                                   const uint8_t* __restrict__ mflags, uint8_t* __restrict__ vflags,
+                                                   int clear_v, int nblocks, uint2* __restrict__ list,
+                                                   uint32_t* __restrict__ count) {
+    __shared__ uint32_t wcount[4];
+    __shared__ uint32_t wbase;
+    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t fm = 0u, fv = 0u;
+    if (blk < nblocks) {
+        const DevCfg& c = cfgs[block_scene[blk]];
+        const int bx = (blk - c.foff) % c.fbx;
+        if ((bx & 3) == 0) {
+            const int nk = min(4, c.fbx - bx);
+            for (int k = 0; k < nk; ++k) {
+                if (mflags[blk + k]) fm |= 1u << k;
+                if (clear_v && vflags[blk + k]) {
+                    fv |= 1u << k;
+                    vflags[blk + k] = 0;
+                }
+            }
         }
     }
-    if (fv && lane == 0) v.flags[blk] = 0;
+    const bool any = (fm | fv) != 0u;
+    const uint64_t m = __ballot(any);
+    if (lane == 0) wcount[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t tot = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        wbase = tot ? atomicAdd(count, tot) : 0u;
+    }
+    __syncthreads();
+    if (any) {
+        uint32_t base = wbase;
+        for (int w = 0; w < wave; ++w) base += wcount[w];
+        const uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
+        list[base + __popcll(below)] = make_uint2((uint32_t)blk, fm | (fv << 4));
+    }
+}
+
+__global__ void k_set_u32_grid(uint32_t* p, uint32_t v) { *p = v; }
+
+// 2. One wave per segment at a time: lane = (x 0..15, y 0..3), the four z
+// slices' loads issued together.  single: every block belongs to scene 0.
+__global__ __launch_bounds__(256) void k_grid_update(const DevCfg* __restrict__ cfgs,
+                                                     const uint16_t* __restrict__ block_scene, GridBuf m, GridBuf v,
+                                                     const float* __restrict__ sdf, const uint2* __restrict__ list,
+                                                     const uint32_t* __restrict__ count, int single) {
+    const int n = (int)*count;
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), nw = gridDim.x * 4;
+    DevCfg c0{};
+    if (single) c0 = uniform(cfgs[0]);
+    for (int i = __builtin_amdgcn_readfirstlane(wave); i < n; i += nw) {
+        const uint2 e = uniform(list[i]);
+        const int blk = (int)e.x;
+        const DevCfg c = single ? c0 : uniform(cfgs[block_scene[blk]]);
+        const int lb = blk - c.foff;
+        const int bx = lb % c.fbx, by = (lb / c.fbx) % c.fby, bz = lb / (c.fbx * c.fby);
+        const int k = (lane & 15) >> 2;
+        const bool fm = (e.y >> k) & 1u, fv = (e.y >> (4 + k)) & 1u;
+        const int x = bx * 4 + (lane & 15), y = by * 4 + (lane >> 4);
+        GuNode nd[4];
+#pragma unroll
+        for (int zz = 0; zz < 4; ++zz) {
+            const int z = bz * 4 + zz;
+            const int g = (fm || fv) && inb(c, x, y, z) ? gidx(c, x, y, z) : -1;
+            nd[zz] = grid_update_load(m, sdf, g, fm);
+        }
+#pragma unroll
+        for (int zz = 0; zz < 4; ++zz) grid_update_store(c, m, v, sdf, nd[zz], x, y, bz * 4 + zz, fm, fv);
+    }
 }
 
 // makeSDF, kernels.cu:282-300, one thread per node.  The triangle index is
@@ -292,11 +378,15 @@ void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_copy, dim3(256 * 8 * 4), dim3(256), 0, s, src, dst, n);
 }
 
-void launch_grid_update(const DevCfg* cfgs, const uint16_t* block_scene, GridBuf m, GridBuf v_clear, bool clear_v,
-                        const float* sdf, int nblocks, hipStream_t s) {
+void launch_grid_update(const DevCfg* cfgs, const uint16_t* block_scene, int nscenes, GridBuf m, GridBuf v_clear,
+                        bool clear_v, const float* sdf, int nblocks, uint2* list, uint32_t* count, hipStream_t s) {
     if (nblocks <= 0) return;
-    hipLaunchKernelGGL(k_grid_update, dim3((nblocks + 3) / 4), dim3(256), 0, s, cfgs, block_scene, m, v_clear,
-                       clear_v ? 1 : 0, sdf, nblocks);
+    hipLaunchKernelGGL(k_set_u32_grid, dim3(1), dim3(1), 0, s, count, 0u);
+    hipLaunchKernelGGL(k_grid_list, dim3((nblocks + 255) / 256), dim3(256), 0, s, cfgs, block_scene, m.flags,
+                       v_clear.flags, clear_v ? 1 : 0, nblocks, list, count);
+    const int waves = std::min((nblocks + 3) / 4, 8192);
+    hipLaunchKernelGGL(k_grid_update, dim3((waves + 3) / 4), dim3(256), 0, s, cfgs, block_scene, m, v_clear, sdf,
+                       list, count, nscenes == 1 ? 1 : 0);
 }
 
 void launch_make_sdf(const DevCfg& c, const float* tris, int nt, float* sdf, hipStream_t s) {
@@ -414,12 +504,12 @@ int launch_make_sdf_bvh(const DevCfg& c, const float* tris_dev, const float* tri
     unsigned int* dc = nullptr;
     if (hipMalloc((void**)&dn, b.nodes.size() * sizeof(SdfNode)) != hipSuccess) return -1;
     if (hipMalloc((void**)&dp, pt.size() * sizeof(float)) != hipSuccess) {
-        hipFree(dn);
+        (void)hipFree(dn);
         return -1;
     }
     if (hipMalloc((void**)&dc, sizeof(unsigned int)) != hipSuccess) {
-        hipFree(dn);
-        hipFree(dp);
+        (void)hipFree(dn);
+        (void)hipFree(dp);
         return -1;
     }
     hipError_t e = hipMemcpyAsync(dn, b.nodes.data(), b.nodes.size() * sizeof(SdfNode), hipMemcpyHostToDevice, s);
@@ -434,9 +524,9 @@ int launch_make_sdf_bvh(const DevCfg& c, const float* tris_dev, const float* tri
     unsigned int cnt = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&cnt, dc, sizeof(unsigned int), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    hipFree(dn);
-    hipFree(dp);
-    hipFree(dc);
+    (void)hipFree(dn);
+    (void)hipFree(dp);
+    (void)hipFree(dc);
     if (exact_count) *exact_count = cnt;
     return e == hipSuccess ? 0 : -1;
 }
