@@ -281,39 +281,57 @@ __device__ __forceinline__ uint32_t bucket_key(const DevCfg& c, const uint32_t* 
 
 // 1. keys[i] and the particle's place within its bucket (rank[i]); bcount
 // holds zeros on entry and the bucket sizes on exit.  The scene of slot i
-// comes from its caller index (orig, or i when orig is null).
+// comes from its caller index (orig, or i when orig is null).  A wave takes
+// BC_BATCH runs of 64 consecutive slots and issues their atomics together.
+constexpr int BC_BATCH = 4;
 __global__ __launch_bounds__(256) void k_bucket_count(const float* __restrict__ px, const float* __restrict__ py,
                                                       const float* __restrict__ pz, int n, SceneTable t,
                                                       const int* __restrict__ orig,
                                                       const uint32_t* __restrict__ brank,
                                                       uint32_t* __restrict__ bcount, uint32_t* __restrict__ keys,
                                                       uint32_t* __restrict__ rank) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    const bool valid = i < n;  // the valid lanes of a wave are a prefix
-    uint32_t key = 0u, b = 0xFFFFFFFFu;
-    if (valid) {
-        const int sc = t.nscenes > 1 ? scene_of(t, orig ? orig[i] : i) : 0;
-        key = bucket_key(t.cfgs[sc], brank, px[i], py[i], pz[i]);
-        b = key >> 6;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    uint32_t key[BC_BATCH], b[BC_BATCH], got[BC_BATCH];
+    uint64_t heads[BC_BATCH];
+    int nvalid[BC_BATCH];
+#pragma unroll
+    for (int j = 0; j < BC_BATCH; ++j) {
+        const int i = (wave * BC_BATCH + j) * 64 + lane;
+        const bool valid = i < n;  // the valid lanes of a batch are a prefix
+        key[j] = 0u;
+        b[j] = 0xFFFFFFFFu;
+        if (valid) {
+            const int sc = t.nscenes > 1 ? scene_of(t, orig ? orig[i] : i) : 0;
+            key[j] = bucket_key(t.cfgs[sc], brank, px[i], py[i], pz[i]);
+            b[j] = key[j] >> 6;
+        }
+        const uint32_t prev = __shfl_up(b[j], 1);
+        const bool head = valid && (lane == 0 || prev != b[j]);
+        heads[j] = __ballot(head);
+        nvalid[j] = __popcll(__ballot(valid));
     }
-    const uint32_t prev = __shfl_up(b, 1);
-    const bool head = valid && (lane == 0 || prev != b);
-    const uint64_t heads = __ballot(head);
-    const int nvalid = __popcll(__ballot(valid));
-    uint32_t got = 0u;
-    if (head) {
-        const uint64_t later = heads & ~lanemask_le(lane);
-        const int end = later ? __ffsll((unsigned long long)later) - 1 : nvalid;
-        got = atomicAdd(bcount + b, (uint32_t)(end - lane));
+#pragma unroll
+    for (int j = 0; j < BC_BATCH; ++j) {
+        got[j] = 0u;
+        if ((heads[j] >> lane) & 1ull) {
+            const uint64_t later = heads[j] & ~lanemask_le(lane);
+            const int end = later ? __ffsll((unsigned long long)later) - 1 : nvalid[j];
+            got[j] = atomicAdd(bcount + b[j], (uint32_t)(end - lane));
+        }
     }
-    // the head of a valid lane's run: the highest head at or below it
-    const uint64_t mine = heads & lanemask_le(lane);
-    const int hl = valid ? 63 - __clzll((long long)mine) : lane;
-    const uint32_t base = __shfl(got, hl);
-    if (valid) {
-        keys[i] = key;
-        rank[i] = base + (uint32_t)(lane - hl);
+#pragma unroll
+    for (int j = 0; j < BC_BATCH; ++j) {
+        const int i = (wave * BC_BATCH + j) * 64 + lane;
+        const bool valid = i < n;
+        // the head of a valid lane's run: the highest head at or below it
+        const uint64_t mine = heads[j] & lanemask_le(lane);
+        const int hl = valid ? 63 - __clzll((long long)mine) : lane;
+        const uint32_t base = __shfl(got[j], hl);
+        if (valid) {
+            keys[i] = key[j];
+            rank[i] = base + (uint32_t)(lane - hl);
+        }
     }
 }
 
@@ -383,7 +401,9 @@ constexpr int RO_THREADS = 256;
 // r takes the r-th member of every cell with more than r -- so the lanes of a
 // round hold distinct base cells.  Then map[new slot] = old slot, and the
 // caller indices and the planes in `mask` (bit a = attribute a) move to the
-// new slots.  Dynamic LDS: 3 x cap words.
+// new slots (PLANES = false: mask is 0, only the map and the caller indices;
+// fewer registers, more chunks in flight).  Dynamic LDS: 3 x cap words.
+template <bool PLANES>
 __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ ch,
                                                         const uint32_t* __restrict__ counters, int cap,
                                                         const uint32_t* __restrict__ keys,
@@ -407,9 +427,14 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
         const ChunkDesc d = uniform(ch[ci]);
         if (tid < 64) cnt[tid] = 0u;
         __syncthreads();
+#pragma unroll 4
         for (int q = tid; q < d.count; q += RO_THREADS) {
-            const uint32_t c = keys[d.start + q] & 63u;
+            cell_rank[q] = keys[d.start + q] & 63u;
             slot_of[q] = vals[d.start + q];
+        }
+        __syncthreads();
+        for (int q = tid; q < d.count; q += RO_THREADS) {
+            const uint32_t c = cell_rank[q];
             cell_rank[q] = c | (atomicAdd(&cnt[c], 1u) << 6);
         }
         __syncthreads();
@@ -453,13 +478,15 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
             const uint32_t si = slot_of[q];
             const size_t o = (size_t)d.start + q;
             map[o] = si;
-            float v[A_STATE];
+            if constexpr (PLANES) {
+                float v[A_STATE];
 #pragma unroll
-            for (int a = 0; a < A_STATE; ++a)
-                if (mask & (1u << a)) v[a] = src[(size_t)a * src_stride + si];
+                for (int a = 0; a < A_STATE; ++a)
+                    if (mask & (1u << a)) v[a] = src[(size_t)a * src_stride + si];
 #pragma unroll
-            for (int a = 0; a < A_STATE; ++a)
-                if (mask & (1u << a)) dst[(size_t)a * dst_stride + o] = v[a];
+                for (int a = 0; a < A_STATE; ++a)
+                    if (mask & (1u << a)) dst[(size_t)a * dst_stride + o] = v[a];
+            }
             dst_orig[o] = src_orig ? src_orig[si] : (int)si;
         }
         __syncthreads();  // the LDS arrays are reused by the next chunk
@@ -525,7 +552,8 @@ void launch_resort(const ResortArgs& a, hipStream_t s) {
     const unsigned blocks = (unsigned)((n + 255) / 256);
     const BlockTable& bt = a.blocks;
     if (n > 0)
-        hipLaunchKernelGGL(k_bucket_count, dim3(blocks), dim3(256), 0, s, a.src + (size_t)A_PX * a.src_stride,
+        hipLaunchKernelGGL(k_bucket_count, dim3((unsigned)((n + 256 * BC_BATCH - 1) / (256 * BC_BATCH))), dim3(256),
+                           0, s, a.src + (size_t)A_PX * a.src_stride,
                            a.src + (size_t)A_PY * a.src_stride, a.src + (size_t)A_PZ * a.src_stride, n, a.scenes,
                            a.src_orig, bt.rank, bt.count, a.keys[0], a.vals[0]);
     scan_exclusive<uint64_t>(LoadBucket{bt.count, (uint32_t)a.chunk_cap}, bt.scan, (size_t)bt.nb, bt.partial,
@@ -537,10 +565,16 @@ void launch_resort(const ResortArgs& a, hipStream_t s) {
                        bt.block, a.block_scene, a.scenes.cfgs, bt.nb, a.chunk_cap, a.counters, a.chunks,
                        a.chunk_limit);
     if (n <= 0) return;
-    const int grid = std::max(1, std::min(a.chunk_est, 8192));
-    hipLaunchKernelGGL(k_reorder, dim3((unsigned)grid), dim3(RO_THREADS), (size_t)3 * a.chunk_cap * sizeof(uint32_t),
-                       s, a.chunks, a.counters, a.chunk_cap, a.keys[1], a.vals[1], a.src, a.src_stride, a.src_orig,
-                       a.dst, a.dst_stride, a.dst_orig, a.rounds, a.counters + 4, a.map, a.mask);
+    const int grid = std::max(1, std::min(a.chunk_est, 16384));
+    const size_t lds = (size_t)3 * a.chunk_cap * sizeof(uint32_t);
+    if (a.mask)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_reorder<true>), dim3((unsigned)grid), dim3(RO_THREADS), lds, s, a.chunks,
+                           a.counters, a.chunk_cap, a.keys[1], a.vals[1], a.src, a.src_stride, a.src_orig, a.dst,
+                           a.dst_stride, a.dst_orig, a.rounds, a.counters + 4, a.map, a.mask);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_reorder<false>), dim3((unsigned)grid), dim3(RO_THREADS), lds, s, a.chunks,
+                           a.counters, a.chunk_cap, a.keys[1], a.vals[1], a.src, a.src_stride, a.src_orig, a.dst,
+                           a.dst_stride, a.dst_orig, a.rounds, a.counters + 4, a.map, 0u);
 }
 
 }  // namespace mpm
