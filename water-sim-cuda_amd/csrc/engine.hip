@@ -96,7 +96,8 @@ struct mpm_engine {
     int chunk_est = 0;    // chunk count at the last host synchronisation (sizes launches)
 
     GridBuf grid[2] = {};
-    uint2* grid_list = nullptr;  // nfb: flagged row segments of a grid update
+    uint2* grid_list = nullptr;  // max_segs: flagged row segments of a grid update
+    int max_segs = 0;            // row segments (four x-adjacent 4^3 blocks) of all scenes
     int vcur = 0;  // grid[vcur]: velocities of the last update; grid[1 - vcur]: accumulates
     float* sdf = nullptr;
 
@@ -400,7 +401,8 @@ void run_grid_update(mpm_engine* h) {
     if (h->correct && h->corr && h->ccap > 0) {
         StageTimer t(h, ST_CORR);
         launch_set_u32(corr_count(h), 0, 0u, h->stream);
-        launch_corr_active(h->d_cfgs, h->d_block_scene, V, h->nfb, h->clist, h->ccap, corr_count(h), h->stream);
+        launch_corr_active_seg(h->d_cfgs, h->d_block_scene, V, h->grid_list, h->sc.counters + 14, h->max_segs,
+                               h->clist, h->ccap, corr_count(h), h->stream);
         launch_corr_correct(h->d_cfgs, V, h->sdf, h->clist, corr_count(h), h->ccap, *h->corr, h->cY, h->stream);
         launch_corr_apply(h->d_cfgs, V, h->clist, corr_count(h), h->ccap, h->cY, h->stream);
     }
@@ -600,6 +602,7 @@ int create_engine(const mpm_sim_config* cfgs_in, int S, mpm_engine** out) {
     if (nfb >= (1ll << 26)) return cleanup(fail(MPM_ELIMIT, "scenes hold 2^26 or more 4^3 blocks in all"));
     h->G = (size_t)G;
     h->nfb = (int)nfb;
+    for (const DevCfg& d : h->dcs) h->max_segs += d.fby * d.fbz * ((d.fbx + 3) / 4);
     if (hipGetDevice(&h->device) != hipSuccess) return cleanup(fail(MPM_EHIP, "no HIP device"));
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(MPM_EHIP, "hipStreamCreate failed"));
@@ -982,11 +985,11 @@ int mpm_engine_check(mpm_engine* h, float depth, double* out3) {
     int rc = check_engine(h, true);
     if (rc) return rc;
     if (!out3) return fail(MPM_EINVAL, "null output");
-    if (!h->check_buf && (rc = dev_alloc(&h->check_buf, 4))) return rc;
-    HIPCHK(hipMemsetAsync(h->check_buf, 0, 4 * sizeof(float), h->stream));
+    if (!h->check_buf && (rc = dev_alloc(&h->check_buf, 3 + 3 * (size_t)CHECK_BLOCKS))) return rc;
+    HIPCHK(hipMemsetAsync(h->check_buf, 0, 3 * sizeof(float), h->stream));
     launch_check(h->planes[h->pcur], h->plane_cap[h->pcur], h->orig[h->pcur], h->n, scene_table(h), h->sdf, depth,
                  h->check_buf, h->stream);
-    float v[4] = {0, 0, 0, 0};
+    float v[3] = {0, 0, 0};
     HIPCHK(hipMemcpyAsync(v, h->check_buf, sizeof(v), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     out3[0] = v[0];

@@ -172,6 +172,10 @@ void corr_pack(const float* params, void* frags, hipStream_t s);
 // the device, at most cap entries stored).
 void launch_corr_active(const DevCfg* cfgs, const uint16_t* block_scene, GridBuf V, int nblocks, int2* list, int cap,
                         uint32_t* count, hipStream_t s);
+// The same list from the grid update's flagged row segments (segs, *nsegs as
+// left by launch_grid_update; max_segs bounds their number).
+void launch_corr_active_seg(const DevCfg* cfgs, const uint16_t* block_scene, GridBuf V, const uint2* segs,
+                            const uint32_t* nsegs, int max_segs, int2* list, int cap, uint32_t* count, hipStream_t s);
 void launch_corr_features(const DevCfg* cfgs, GridBuf V, const float* sdf, const int2* list, const uint32_t* count,
                           int cap, float* X, hipStream_t s);
 void launch_corr_forward(const CorrState& cs, const float* X, int cap, const uint32_t* count, float* Y, hipStream_t s);
@@ -185,6 +189,9 @@ void launch_corr_apply(const DevCfg* cfgs, GridBuf V, const int2* list, const ui
 void launch_corr_gather_vel(const DevCfg* cfgs, GridBuf V, const int2* list, const uint32_t* count, int cap, float* out,
                             float* hits, int mode, hipStream_t s);
 // out3 += {particles below SDF -depth, sum |det F - 1|, non-finite particles}.
+// Fallback checks: out3 must hold 3 + 3 * CHECK_BLOCKS floats (per-workgroup
+// partials after the three sums).
+constexpr int CHECK_BLOCKS = 1024;
 void launch_check(const float* part, size_t stride, const int* orig, int n, const SceneTable& t, const float* sdf,
                   float depth, float* out3, hipStream_t s);
 
