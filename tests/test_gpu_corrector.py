@@ -121,6 +121,44 @@ def test_features_match_oracle(gpu):
     e.close()
 
 
+def test_engine_correction_matches_forward(gpu):
+    """The engine's fused feature + MLP kernel against the feature kernel and
+    the standalone forward pass (both checked against the oracle above), with
+    random weights: one substep's grid velocity with the corrector minus the
+    same substep's without it equals forward(features) at every active node
+    (within BF16 units of the output scale; the two engines differ only by
+    float-atomic order)."""
+    sc = gpu.scenes.sphere64(physical=True)
+    a, b = gpu.Engine(sc["cfg"]), gpu.Engine(sc["cfg"])
+    for e in (a, b):
+        e.init(sc["particles"].shape[1], sc["particles"])
+        e.set_sdf(sc["sdf"])
+        e.steps(10)
+    a.steps(1)
+    nodes, X = a.debug_features(300000)
+    assert len(nodes) > 1000
+    c = gpu.Corrector(mlp.init_params(3, 2.0).astype(np.float32))
+    mean = X[:, :66].mean(0).astype(np.float32)
+    inv = (1.0 / (X[:, :66].std(0) + 1e-3)).astype(np.float32)
+    c.set_norm(mean, inv, 0.5)
+    c.set_clamp(1e6)
+    b.set_corrector(c)
+    b.set_option("CORRECT", 1)
+    b.steps(1)
+    want = c.forward(X)
+    ga, gb = a.getGrid(), b.getGrid()
+    sx, sy = int(sc["cfg"].SIZE_X), int(sc["cfg"].SIZE_Y)
+    code = nodes[:, 0].astype(np.int64)
+    g = (code & 1023) + sx * (((code >> 10) & 1023) + sy * (code >> 20))
+    got = (gb[1:4, g] - ga[1:4, g]).T
+    scale = float(np.abs(want).max())
+    err = float(np.abs(got - want).max())
+    say(f"engine correction vs forward: {len(nodes)} nodes, max |correction| {scale:.3g}, max error {err:.3g}")
+    assert scale > 1e-2
+    assert err < 0.03 * scale
+    a.close(); b.close(); c.close()
+
+
 def test_zero_corrector_changes_nothing(gpu):
     sc = gpu.scenes.sphere64(physical=True)
     a = gpu.Engine(sc["cfg"])

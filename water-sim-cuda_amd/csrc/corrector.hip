@@ -531,50 +531,67 @@ __global__ __launch_bounds__(256) void k_features(const DevCfg* __restrict__ cfg
     }
 }
 
-// Features and forward pass fused: the corrections Y[row] of the listed
-// nodes, with each lane computing its node's feature row in registers (the
-// two lanes of a node both compute it and keep their halves), so no feature
-// rows travel through memory.  Weight fragments are staged in LDS.
+// Fused features + forward over the active-node list.  Each wave takes 64
+// nodes at a time, one per lane: the lane computes its node's features and
+// stores them normalised, in BF16, in the first layer's B-fragment order in
+// LDS (node r of tile t: K-block kb's features 8kb..8kb+3 at lane r, 8kb+4..
+// 8kb+7 at lane r + 32); the wave then runs the MLP on its two 32-node tiles.
+// Four waves per workgroup share the forward weight fragments in LDS.
 __global__ __launch_bounds__(256) void k_correct(const DevCfg* __restrict__ cfgs, GridBuf V,
                                                  const float* __restrict__ sdf, const int2* __restrict__ list,
                                                  const uint32_t* __restrict__ count, int cap, Norm nm,
                                                  const bf16x4* __restrict__ fr, const float* __restrict__ P,
                                                  float* __restrict__ Y) {
     __shared__ bf16x4 wfr[FWD_FRAGS * 64];
+    __shared__ bf16x4 xin[4][2][9][64];  // [wave][tile][K-block][lane]
     for (int i = threadIdx.x; i < FWD_FRAGS * 64; i += blockDim.x) wfr[i] = fr[i];
     __syncthreads();
     const int rows = (int)min(*count, (uint32_t)cap);
     const int lane = threadIdx.x & 63, hi = lane >> 5, wave = threadIdx.x >> 6;
-    for (int tile = blockIdx.x * 4 + wave; tile * 32 < rows; tile += gridDim.x * 4) {
-        const int row = tile * 32 + (lane & 31);
-        const bool valid = row < rows;
-        float f[KIN];
-        if (valid) {
-            const int2 e = list[row];
-            node_features(cfgs[e.y], V, sdf, e.x & 1023, (e.x >> 10) & 1023, e.x >> 20, f);
-        } else {
+    bf16x4 (*xs)[9][64] = xin[wave];
+    for (int base = (blockIdx.x * 4 + wave) * 64; base < rows; base += gridDim.x * 4 * 64) {
+        {
+            const int row = base + lane;
+            float f[KIN];
+            if (row < rows) {
+                const int2 e = list[row];
+                node_features(cfgs[e.y], V, sdf, e.x & 1023, (e.x >> 10) & 1023, e.x >> 20, f);
+            } else {
 #pragma unroll
-            for (int i = 0; i < KIN; ++i) f[i] = nm.mean[i];  // normalises to 0
-        }
-        bf16x4 xb[9], h1[8], h2[8], h3[8];
-#pragma unroll
-        for (int kb = 0; kb < 9; ++kb) {
-            float v[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int k0 = kb * 8 + i, k1 = k0 + 4;
-                v[i] = hi ? (f[k1] - nm.mean[k1]) * nm.inv_std[k1] : (f[k0] - nm.mean[k0]) * nm.inv_std[k0];
+                for (int i = 0; i < KIN; ++i) f[i] = nm.mean[i];  // normalises to 0
             }
-            xb[kb] = pack4(v[0], v[1], v[2], v[3]);
+            const int t = lane >> 5, r = lane & 31;
+#pragma unroll
+            for (int kb = 0; kb < 9; ++kb)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    float v[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int k = kb * 8 + 4 * h + i;
+                        v[i] = (f[k] - nm.mean[k]) * nm.inv_std[k];
+                    }
+                    xs[t][kb][r + 32 * h] = pack4(v[0], v[1], v[2], v[3]);
+                }
         }
-        f32x16 a4;
-        asm volatile("" ::: "memory");  // weight fragments are re-read from LDS per tile
-        mlp_fwd(wfr, P, xb, lane, h1, h2, h3, a4);
-        if (valid && hi == 0) {
-            Y[(size_t)row * 3 + 0] = fminf(fmaxf(a4[0], -nm.clamp), nm.clamp) * nm.out_scale;
-            Y[(size_t)row * 3 + 1] = fminf(fmaxf(a4[1], -nm.clamp), nm.clamp) * nm.out_scale;
-            Y[(size_t)row * 3 + 2] = fminf(fmaxf(a4[2], -nm.clamp), nm.clamp) * nm.out_scale;
+        // the wave's LDS operations run in order: the reads below see the writes
+        asm volatile("" ::: "memory");
+        for (int tt = 0; tt < 2; ++tt) {
+            if (base + tt * 32 >= rows) break;
+            bf16x4 xb[9], h1[8], h2[8], h3[8];
+#pragma unroll
+            for (int kb = 0; kb < 9; ++kb) xb[kb] = xs[tt][kb][lane];
+            f32x16 a4;
+            asm volatile("" ::: "memory");  // weight fragments are re-read from LDS per tile
+            mlp_fwd(wfr, P, xb, lane, h1, h2, h3, a4);
+            const int row = base + tt * 32 + (lane & 31);
+            if (row < rows && hi == 0) {
+                Y[(size_t)row * 3 + 0] = fminf(fmaxf(a4[0], -nm.clamp), nm.clamp) * nm.out_scale;
+                Y[(size_t)row * 3 + 1] = fminf(fmaxf(a4[1], -nm.clamp), nm.clamp) * nm.out_scale;
+                Y[(size_t)row * 3 + 2] = fminf(fmaxf(a4[2], -nm.clamp), nm.clamp) * nm.out_scale;
+            }
         }
+        asm volatile("" ::: "memory");  // the next nodes overwrite xs
     }
 }
 
