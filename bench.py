@@ -64,7 +64,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2000, help="timed substeps")
     ap.add_argument("--warmup", type=int, default=80, help="untimed substeps")
     ap.add_argument("--scene", default="channel256")
-    ap.add_argument("--resort-every", type=int, default=None)
+    ap.add_argument("--resort-every", type=int, default=None, help="re-sort period (0: adaptive) [the scene's]")
     ap.add_argument("--chunk", type=int, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
@@ -106,7 +106,7 @@ def main() -> int:
         cfg = sc["cfg"]
         n = sc["particles"].shape[1]
         grid_desc = [cfg.SIZE_X, cfg.SIZE_Y, cfg.SIZE_Z]
-    resort = args.resort_every or sc["resort_every"]
+    resort = sc["resort_every"] if args.resort_every is None else args.resort_every
     log(f"scene {sc['name']}: n={n} grid={grid_desc} (host gen {time.time() - t0:.1f}s)")
 
     opts = {"RESORT_EVERY": resort, "TIMING": 0 if args.no_timing else 1, "GRAPH": 1 if args.graph else 0}
