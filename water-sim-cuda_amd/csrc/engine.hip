@@ -1095,13 +1095,10 @@ int mpm_engine_snapshot(mpm_engine* h) {
         if ((rc = dev_alloc(&h->snap_planes, (size_t)A_STATE * h->n)) || (rc = dev_alloc(&h->snap_orig, (size_t)h->n)))
             return rc;
     }
-    const size_t st = h->plane_cap[h->pcur];
-    for (int a = 0; a < A_STATE; ++a)
-        HIPCHK(hipMemcpyAsync(h->snap_planes + (size_t)a * h->n, h->planes[h->pcur] + (size_t)a * st,
-                              (size_t)h->n * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
-    HIPCHK(hipMemcpyAsync(h->snap_orig, h->orig[h->pcur], (size_t)h->n * sizeof(int), hipMemcpyDeviceToDevice,
-                          h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    // stream-ordered: whatever runs next on the engine's stream sees the copy
+    launch_copy_state(h->planes[h->pcur], h->plane_cap[h->pcur], h->orig[h->pcur], h->snap_planes, (size_t)h->n,
+                      h->snap_orig, h->n, A_STATE, h->stream);
+    HIPCHK(hipGetLastError());
     h->snap_substeps = h->substeps;
     return MPM_OK;
 }
@@ -1112,12 +1109,9 @@ int mpm_engine_restore(mpm_engine* h) {
     if (h->n == 0) return MPM_OK;
     if (h->snap_substeps < 0) return fail(MPM_ESTATE, "no snapshot taken");
     drop_graph(h);
-    const size_t st = h->plane_cap[h->pcur];
-    for (int a = 0; a < A_STATE; ++a)
-        HIPCHK(hipMemcpyAsync(h->planes[h->pcur] + (size_t)a * st, h->snap_planes + (size_t)a * h->n,
-                              (size_t)h->n * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
-    HIPCHK(hipMemcpyAsync(h->orig[h->pcur], h->snap_orig, (size_t)h->n * sizeof(int), hipMemcpyDeviceToDevice,
-                          h->stream));
+    launch_copy_state(h->snap_planes, (size_t)h->n, h->snap_orig, h->planes[h->pcur], h->plane_cap[h->pcur],
+                      h->orig[h->pcur], h->n, A_STATE, h->stream);
+    HIPCHK(hipGetLastError());
     h->substeps = h->snap_substeps;
     // The saved slots may belong to another chunk layout: lay them out anew.
     if ((rc = resort(h, true))) return rc;

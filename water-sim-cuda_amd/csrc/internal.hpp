@@ -65,6 +65,9 @@ int launch_make_sdf_bvh(const DevCfg& c, const float* tris_dev, const float* tri
                         unsigned int* exact_count, hipStream_t s);
 void launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
 void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s);
+// Copies `planes` particle planes (n slots) and the caller indices.
+void launch_copy_state(const float* src, size_t src_stride, const int* src_orig, float* dst, size_t dst_stride,
+                       int* dst_orig, int n, int planes, hipStream_t s);
 void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t s);
 void launch_init_F(float* soa26, int n, hipStream_t s);
 

@@ -364,6 +364,18 @@ __global__ __launch_bounds__(256) void k_export_positions(const float* __restric
     }
 }
 
+// dst plane a, slot i = src plane a, slot i for a < planes, and the caller
+// indices (one launch for a whole particle state).
+__global__ __launch_bounds__(256) void k_copy_state(const float* __restrict__ src, size_t src_stride,
+                                                    const int* __restrict__ src_orig, float* __restrict__ dst,
+                                                    size_t dst_stride, int* __restrict__ dst_orig, int n,
+                                                    int planes) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        for (int a = 0; a < planes; ++a) dst[(size_t)a * dst_stride + i] = src[(size_t)a * src_stride + i];
+        dst_orig[i] = src_orig[i];
+    }
+}
+
 // Streaming copy (measurement aid: calibrates HBM byte counters and gives the
 // achievable copy bandwidth).
 __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
@@ -372,6 +384,14 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
 }
 
 }  // namespace
+
+void launch_copy_state(const float* src, size_t src_stride, const int* src_orig, float* dst, size_t dst_stride,
+                       int* dst_orig, int n, int planes, hipStream_t s) {
+    if (n <= 0) return;
+    const int blocks = std::min((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_copy_state, dim3(blocks), dim3(256), 0, s, src, src_stride, src_orig, dst, dst_stride,
+                       dst_orig, n, planes);
+}
 
 void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s) {
     if (!n) return;
