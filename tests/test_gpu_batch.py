@@ -71,6 +71,29 @@ def test_batch_matches_single_scene_engines(gpu, small_batch):
     eb.close()
 
 
+def test_odd_size_batch_matches_single_scene_engines(gpu):
+    """Scenes whose edges (21, 37, 45) give block counts that are not
+    multiples of 4: the block numbering, the chunk table and the grid
+    update's row segments cross scene boundaries in the middle of a segment
+    and with partial blocks."""
+    b = gpu.scenes.batch_scenes(count=5, seed=23, n_min=2000, n_max=40000, edges=(21, 37, 45))
+    say(f"odd batch: edges {[c.SIZE_X for c in b['cfgs']]}, n={int(b['counts'].sum())}")
+    eb = batch_engine(gpu, b, RESORT_EVERY=3)
+    eb.steps(8)
+    got = eb.getParticles()
+    for s, cfg in enumerate(b["cfgs"]):
+        lo, hi = scene_slice(b, s)
+        e = gpu.Engine(cfg, RESORT_EVERY=3)
+        e.init(hi - lo, np.ascontiguousarray(b["particles"][:, lo:hi]))
+        e.set_sdf(b["sdfs"][s])
+        e.steps(8)
+        ref = e.getParticles()
+        e.close()
+        assert max_err(got[0:3, lo:hi], ref[0:3]) < 8 * POS_TOL, s
+        assert max_err(got[3:6, lo:hi], ref[3:6]) < 8 * VEL_TOL, s
+    eb.close()
+
+
 def test_batch_matches_oracle(gpu, oracle, small_batch):
     b = small_batch
     say("batch vs oracle: one substep per scene")
