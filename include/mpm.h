@@ -246,6 +246,14 @@ int mpm_engine_debug_grid_update(mpm_engine* h, const float* grid4_in, float* gr
  * mass/velocity grid; updates the particle state in place. */
 int mpm_engine_debug_g2p(mpm_engine* h, const float* grid4_in);
 
+/* ---- page-locked host memory (no reference counterpart) ----
+ * Read-backs (get_positions, get_particles, get_grid) into page-locked host
+ * memory run at the full host-link rate; into ordinary (pageable) memory the
+ * HIP runtime stages them at a fraction of it.  The reference hands positions
+ * to OpenGL through CUDA-GL interop instead (kernels.cu:341-347). */
+int mpm_host_alloc(size_t bytes, void** out);
+int mpm_host_free(void* p);
+
 /* ---- measurement aid (no reference counterpart) ---- */
 
 /* Copies `bytes` (rounded down to 16) between two fresh device buffers

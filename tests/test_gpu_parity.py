@@ -275,6 +275,12 @@ def test_positions_export(gpu, sphere_phys):
     e.steps(2)
     xyz = e.getPositions()
     np.testing.assert_array_equal(xyz.T, e.getParticles()[0:3])
+    # the same read-back into page-locked host memory
+    pin = gpu.pinned_array(xyz.shape)
+    assert e.getPositions(out=pin) is pin
+    np.testing.assert_array_equal(pin, xyz)
+    with pytest.raises(ValueError):
+        e.getPositions(out=np.empty((3, xyz.shape[0]), np.float32))
     e.close()
 
 

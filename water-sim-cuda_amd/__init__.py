@@ -162,6 +162,8 @@ def load_library() -> ctypes.CDLL:
         "mpm_batch_get_grid": (ctypes.c_int, [eng, ctypes.c_int, c_float_p]),
         "mpm_batch_scene_info": (ctypes.c_int, [eng, ctypes.c_int, c_i64_p]),
         "mpm_probe_copy": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, c_dbl_p]),
+        "mpm_host_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+        "mpm_host_free": (ctypes.c_int, [ctypes.c_void_p]),
         "mpm_obj_triangles": (ctypes.c_int, [ctypes.c_char_p, c_float_p, ctypes.c_int, c_int_p]),
         "mpm_voxelize_obj": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_float, c_float_p, ctypes.c_int64, c_i64_p]),
         "mpm_obj_water": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_float, ctypes.c_float, c_float_p, ctypes.c_int,
@@ -487,9 +489,15 @@ class Engine:
         _check(self._lib.mpm_engine_get_particles(self._h, _fptr(out)))
         return out
 
-    def getPositions(self) -> np.ndarray:
+    def getPositions(self, out=None) -> np.ndarray:
+        """Positions (n, 3) in caller order; into `out` when given (a C-contiguous
+        float32 (n, 3) array, e.g. pinned_array((n, 3)) for the full host-link
+        rate)."""
         n = self.getNumber()
-        out = np.empty((n, 3), dtype=np.float32)
+        if out is None:
+            out = np.empty((n, 3), dtype=np.float32)
+        elif out.shape != (n, 3) or out.dtype != np.float32 or not out.flags.c_contiguous:
+            raise ValueError("out must be a C-contiguous float32 array of shape (n, 3)")
         _check(self._lib.mpm_engine_get_positions(self._h, _fptr(out)))
         return out
 
@@ -662,6 +670,32 @@ class Corrector:
         out = np.empty(steps, dtype=np.float32)
         _check(self._lib.mpm_corrector_train(self._h, int(steps), int(batch), float(lr), _fptr(out)))
         return out
+
+
+class _PinnedBlock:
+    """Owns one mpm_host_alloc block; freed when the last array using it goes."""
+
+    def __init__(self, nbytes: int):
+        self.lib = load_library()
+        self.ptr = ctypes.c_void_p()
+        _check(self.lib.mpm_host_alloc(int(nbytes), ctypes.byref(self.ptr)))
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            self.lib.mpm_host_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+
+def pinned_array(shape, dtype=np.float32) -> np.ndarray:
+    """A numpy array in page-locked host memory: read-backs into it (for
+    example getPositions(out=...)) run at the full host-link rate.  The
+    memory is freed with the last array that uses it."""
+    dt = np.dtype(dtype)
+    nbytes = max(1, int(np.prod(shape)) * dt.itemsize)
+    block = _PinnedBlock(nbytes)
+    buf = (ctypes.c_byte * nbytes).from_address(block.ptr.value)
+    buf._owner = block  # the array's base keeps the block alive
+    return np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
 
 
 def probe_copy(nbytes: int, iters: int = 20) -> float:

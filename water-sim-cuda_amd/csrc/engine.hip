@@ -138,6 +138,8 @@ struct mpm_engine {
     int ccap = 0;
     float dt_scale = 1.0f;
     float* check_buf = nullptr;
+    float* xbuf = nullptr;      // device staging of the read-back calls
+    size_t xbuf_cap = 0;
 
     // Saved particle state (mpm_engine_snapshot / mpm_engine_restore).
     float* snap_planes = nullptr;  // A_STATE x n
@@ -788,6 +790,8 @@ int mpm_engine_destroy(mpm_engine* h) {
     dev_free(h->cacc);
     dev_free(h->chits);
     dev_free(h->check_buf);
+    dev_free(h->xbuf);
+    h->xbuf_cap = 0;
     dev_free(h->d_cfgs);
     dev_free(h->d_block_scene);
     dev_free(h->grid_list);
@@ -1131,19 +1135,31 @@ int mpm_engine_synchronize(mpm_engine* h) {
 
 int mpm_engine_get_number(mpm_engine* h) { return h ? h->n : 0; }
 
+// Device staging for read-backs, kept between calls (grows as needed).
+int export_buffer(mpm_engine* h, size_t floats, float** out) {
+    if (h->xbuf_cap < floats) {
+        dev_free(h->xbuf);
+        h->xbuf_cap = 0;
+        int rc;
+        if ((rc = dev_alloc(&h->xbuf, floats))) return rc;
+        h->xbuf_cap = floats;
+    }
+    *out = h->xbuf;
+    return MPM_OK;
+}
+
 int mpm_engine_get_particles(mpm_engine* h, float* out) {
     int rc = check_engine(h, true);
     if (rc) return rc;
     if (!out) return fail(MPM_EINVAL, "null output");
     if (h->n == 0) return MPM_OK;
     float* tmp = nullptr;
-    if ((rc = dev_alloc(&tmp, (size_t)MPM_PARTICLE_ATTRS * h->n))) return rc;
+    if ((rc = export_buffer(h, (size_t)MPM_PARTICLE_ATTRS * h->n, &tmp))) return rc;
     launch_export_soa26(h->planes[h->pcur], h->plane_cap[h->pcur], h->orig[h->pcur], h->n, scene_table(h), tmp,
                         h->stream);
     hipError_t e = hipMemcpyAsync(out, tmp, sizeof(float) * MPM_PARTICLE_ATTRS * (size_t)h->n, hipMemcpyDeviceToHost,
                                   h->stream);
     hipError_t e2 = hipStreamSynchronize(h->stream);
-    dev_free(tmp);
     if (e != hipSuccess || e2 != hipSuccess) return fail(MPM_EHIP, "get_particles copy failed");
     return MPM_OK;
 }
@@ -1154,18 +1170,11 @@ int mpm_engine_get_positions(mpm_engine* h, float* xyz) {
     if (!xyz) return fail(MPM_EINVAL, "null output");
     if (h->n == 0) return MPM_OK;
     float* tmp = nullptr;
-    if ((rc = dev_alloc(&tmp, (size_t)3 * h->n))) return rc;
-    launch_export_positions(h->planes[h->pcur], h->plane_cap[h->pcur], h->orig[h->pcur], h->n, tmp, h->stream);
-    std::vector<float> planes((size_t)3 * h->n);
-    hipError_t e = hipMemcpyAsync(planes.data(), tmp, sizeof(float) * 3 * (size_t)h->n, hipMemcpyDeviceToHost, h->stream);
+    if ((rc = export_buffer(h, (size_t)3 * h->n, &tmp))) return rc;
+    launch_export_xyz(h->planes[h->pcur], h->plane_cap[h->pcur], h->orig[h->pcur], h->n, tmp, h->stream);
+    hipError_t e = hipMemcpyAsync(xyz, tmp, sizeof(float) * 3 * (size_t)h->n, hipMemcpyDeviceToHost, h->stream);
     hipError_t e2 = hipStreamSynchronize(h->stream);
-    dev_free(tmp);
     if (e != hipSuccess || e2 != hipSuccess) return fail(MPM_EHIP, "get_positions copy failed");
-    for (int i = 0; i < h->n; ++i) {
-        xyz[3 * (size_t)i] = planes[i];
-        xyz[3 * (size_t)i + 1] = planes[(size_t)h->n + i];
-        xyz[3 * (size_t)i + 2] = planes[2 * (size_t)h->n + i];
-    }
     return MPM_OK;
 }
 
@@ -1284,6 +1293,22 @@ int mpm_engine_debug_g2p(mpm_engine* h, const float* in) {
     if (h->n > 0) run_transfer(h, XFER_G2P, transfer_launch_size(h));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
+    return MPM_OK;
+}
+
+int mpm_host_alloc(size_t bytes, void** out) {
+    if (!out) return fail(MPM_EINVAL, "null output");
+    *out = nullptr;
+    if (bytes == 0) return MPM_OK;
+    if (hipHostMalloc(out, bytes, hipHostMallocDefault) != hipSuccess) {
+        *out = nullptr;
+        return fail(MPM_ENOMEM, "hipHostMalloc failed");
+    }
+    return MPM_OK;
+}
+
+int mpm_host_free(void* p) {
+    if (p && hipHostFree(p) != hipSuccess) return fail(MPM_EHIP, "hipHostFree failed");
     return MPM_OK;
 }
 

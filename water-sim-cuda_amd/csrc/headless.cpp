@@ -27,6 +27,7 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -130,6 +131,11 @@ int main(int argc, char** argv) {
         std::cerr << "scene: " << n << " particles, " << nt << " collider triangles, grid " << cfg.SIZE_X << "x"
                   << cfg.SIZE_Y << "x" << cfg.SIZE_Z << ", " << cfg.SUBSTEPS << " substeps per frame\n";
         const int substeps = std::max(1u, cfg.SUBSTEPS);
+        // page-locked staging for the per-frame positions (full host-link rate)
+        void* pinned = nullptr;
+        mpm::check(mpm_host_alloc(sizeof(float) * 3 * (size_t)std::max<int64_t>(n, 1), &pinned));
+        std::unique_ptr<void, int (*)(void*)> pinned_owner(pinned, mpm_host_free);
+        float* pos = static_cast<float*>(pinned);
         double t = t0;
         for (int f = 1; f <= frames; ++f) {
             const auto w0 = std::chrono::steady_clock::now();
@@ -139,7 +145,7 @@ int main(int argc, char** argv) {
             t += (double)substeps * cfg.DT;
             const uint32_t frame = frame0 + (uint32_t)f;
             if (f % every != 0 && f != frames) continue;
-            const std::vector<float> pos = engine.getPositions();
+            mpm::check(mpm_engine_get_positions(engine.handle(), pos));
             double sy = 0.0, sy2 = 0.0, ymin = 1e30, ymax = -1e30;
             for (int64_t i = 0; i < n; ++i) {
                 const double y = pos[3 * (size_t)i + 1];
@@ -166,7 +172,7 @@ int main(int argc, char** argv) {
             char name[64];
             if (snapshots) {
                 std::snprintf(name, sizeof(name), "/frame_%06u.pos", frame);
-                if (!write_file(out + name, "MPMP", (uint32_t)n, frame, t, pos.data(), pos.size()))
+                if (!write_file(out + name, "MPMP", (uint32_t)n, frame, t, pos, 3 * (size_t)n))
                     throw std::runtime_error("cannot write " + out + name);
             }
             if (checkpoint) {

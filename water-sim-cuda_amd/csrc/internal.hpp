@@ -75,6 +75,8 @@ void launch_init_F(float* soa26, int n, hipStream_t s);
 // particle orig[s]).
 void launch_export_soa26(const float* part, size_t stride, const int* orig, int n, const SceneTable& t, float* out26,
                          hipStream_t s);
+// Caller-order AoS positions (3 floats per particle).
+void launch_export_xyz(const float* part, size_t stride, const int* orig, int n, float* out_aos, hipStream_t s);
 void launch_export_positions(const float* part, size_t stride, const int* orig, int n, float* out_planes3,
                              hipStream_t s);
 

@@ -376,6 +376,18 @@ __global__ __launch_bounds__(256) void k_copy_state(const float* __restrict__ sr
     }
 }
 
+// Positions in caller order, AoS (x, y, z per particle): changeFormat's
+// layout (kernels.cu:341-347).
+__global__ __launch_bounds__(256) void k_export_xyz(const float* __restrict__ part, size_t stride,
+                                                    const int* __restrict__ orig, int n, float* __restrict__ out) {
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        const size_t id = (size_t)orig[s];
+        out[3 * id] = part[A_PX * stride + s];
+        out[3 * id + 1] = part[A_PY * stride + s];
+        out[3 * id + 2] = part[A_PZ * stride + s];
+    }
+}
+
 // Streaming copy (measurement aid: calibrates HBM byte counters and gives the
 // achievable copy bandwidth).
 __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
@@ -384,6 +396,12 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
 }
 
 }  // namespace
+
+void launch_export_xyz(const float* part, size_t stride, const int* orig, int n, float* out_aos, hipStream_t s) {
+    if (n <= 0) return;
+    const int blocks = std::min((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_export_xyz, dim3(blocks), dim3(256), 0, s, part, stride, orig, n, out_aos);
+}
 
 void launch_copy_state(const float* src, size_t src_stride, const int* src_orig, float* dst, size_t dst_stride,
                        int* dst_orig, int n, int planes, hipStream_t s) {
