@@ -395,15 +395,19 @@ uint32_t* corr_count(mpm_engine* h) { return h->sc.counters + 12; }
 void run_grid_update(mpm_engine* h) {
     {
         StageTimer t(h, ST_GRID);
+        // the list counters alternate with the grid buffers: [14 + vcur]
+        // now, [15 - vcur] for the next update (cleared by this one)
         launch_grid_update(h->d_cfgs, h->d_block_scene, h->S, h->grid[1 - h->vcur], h->grid[h->vcur], true, h->sdf,
-                           h->nfb, h->grid_list, h->sc.counters + 14, h->stream);
+                           h->nfb, h->grid_list, h->sc.counters + 14 + h->vcur, h->sc.counters + 15 - h->vcur,
+                           h->stream);
         h->vcur = 1 - h->vcur;
     }
+    uint32_t* segs = h->sc.counters + 15 - h->vcur;  // this update's list count
     const GridBuf& V = h->grid[h->vcur];
     if (h->correct && h->corr && h->ccap > 0) {
         StageTimer t(h, ST_CORR);
         launch_set_u32(corr_count(h), 0, 0u, h->stream);
-        launch_corr_active_seg(h->d_cfgs, h->d_block_scene, V, h->grid_list, h->sc.counters + 14, h->max_segs,
+        launch_corr_active_seg(h->d_cfgs, h->d_block_scene, V, h->grid_list, segs, h->max_segs,
                                h->clist, h->ccap, corr_count(h), h->stream);
         launch_corr_correct(h->d_cfgs, V, h->sdf, h->clist, corr_count(h), h->ccap, *h->corr, h->cY, h->stream);
         launch_corr_apply(h->d_cfgs, V, h->clist, corr_count(h), h->ccap, h->cY, h->stream);
@@ -660,6 +664,8 @@ int init_particles_impl(mpm_engine* h, const int* counts, const float* soa26, bo
     for (int b = 0; b < 2; ++b)
         if ((rc = zero_grid(h, b))) return rc;
     h->vcur = 0;
+    // both grid-update list counters start at zero with vcur = 0
+    HIPCHK(hipMemsetAsync(h->sc.counters + 14, 0, 2 * sizeof(uint32_t), h->stream));
     h->n = n;
     h->substeps = 0;
     h->sorted_at = 0;
@@ -1277,7 +1283,7 @@ int mpm_engine_debug_grid_update(mpm_engine* h, const float* in, float* out) {
         HIPCHK(hipMemcpyAsync(h->grid[m].plane[p], in + p * h->G, h->G * sizeof(float), hipMemcpyHostToDevice, h->stream));
     launch_fill_u8(h->grid[m].flags, (size_t)h->nfb, 1, h->stream);
     launch_grid_update(h->d_cfgs, h->d_block_scene, h->S, h->grid[m], h->grid[h->vcur], false, h->sdf, h->nfb,
-                       h->grid_list, h->sc.counters + 14, h->stream);
+                       h->grid_list, h->sc.counters + 13, nullptr, h->stream);
     if ((rc = copy_grid_out(h, m, 0, h->G, out, h->G))) return rc;
     return zero_grid(h, m);
 }

@@ -51,9 +51,13 @@ void launch_transfer(int mode, const TransferArgs& a, int grid_chunks, hipStream
 // Grid update over the 4^3 blocks flagged in `m` (blocks of all scenes,
 // block_scene[b] = scene of block b), zeroing the blocks flagged in `v_clear`
 // (the previous velocity buffer) and clearing those flags.  list (nblocks
-// entries) and *count are scratch.
+// entries) and *count are scratch.  With next_count null *count is cleared
+// first; otherwise *count must be zero already and the update clears
+// *next_count for the next one (two counters used in turn need no clearing
+// launch).
 void launch_grid_update(const DevCfg* cfgs, const uint16_t* block_scene, int nscenes, GridBuf m, GridBuf v_clear,
-                        bool clear_v, const float* sdf, int nblocks, uint2* list, uint32_t* count, hipStream_t s);
+                        bool clear_v, const float* sdf, int nblocks, uint2* list, uint32_t* count,
+                        uint32_t* next_count, hipStream_t s);
 
 // makeSDF of one scene (c: host copy of its DevCfg) into its part of `sdf`.
 void launch_make_sdf(const DevCfg& c, const float* tris, int nt, float* sdf, hipStream_t s);
@@ -87,8 +91,10 @@ struct SortScratch {
     uint32_t* hist;      // 256 * ntiles
     uint32_t* partial;   // scan partials
     // [0] = or, [1] = and, [3] = chunks, [4] = rounds of the last layout,
-    // [6..7] = a 64-bit scan total; the engine keeps its per-call counters at
-    // [8..].
+    // [6..7] = a 64-bit scan total; the engine keeps its counters at [8..]:
+    // [8] tile-overflow particles, [9] extra claim passes, [12] corrector
+    // nodes, [13] a stand-alone grid update's list, [14..15] the grid
+    // update's list (in turn with the grid buffers).
     uint32_t* counters;
     uint32_t* map;       // old slot of each slot after the last re-sort
     size_t cap;          // elements

@@ -116,7 +116,8 @@ __global__ __launch_bounds__(256) void k_grid_list(const DevCfg* __restrict__ cf
         uint32_t base = wbase;
         for (int w = 0; w < wave; ++w) base += wcount[w];
         const uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
-        list[base + __popcll(below)] = make_uint2((uint32_t)blk, fm | (fv << 4));
+        const uint32_t slot = base + (uint32_t)__popcll(below);
+        if (slot < (uint32_t)nblocks) list[slot] = make_uint2((uint32_t)blk, fm | (fv << 4));
     }
 }
 
@@ -127,8 +128,10 @@ __global__ void k_set_u32_grid(uint32_t* p, uint32_t v) { *p = v; }
 __global__ __launch_bounds__(256) void k_grid_update(const DevCfg* __restrict__ cfgs,
                                                      const uint16_t* __restrict__ block_scene, GridBuf m, GridBuf v,
                                                      const float* __restrict__ sdf, const uint2* __restrict__ list,
-                                                     const uint32_t* __restrict__ count, int single) {
-    const int n = (int)*count;
+                                                     const uint32_t* __restrict__ count,
+                                                     uint32_t* __restrict__ next_count, int single, int cap) {
+    const int n = (int)min(*count, (uint32_t)cap);
+    if (next_count && blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0u;  // the next update's list starts empty
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), nw = gridDim.x * 4;
     DevCfg c0{};
@@ -417,14 +420,15 @@ void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s) {
 }
 
 void launch_grid_update(const DevCfg* cfgs, const uint16_t* block_scene, int nscenes, GridBuf m, GridBuf v_clear,
-                        bool clear_v, const float* sdf, int nblocks, uint2* list, uint32_t* count, hipStream_t s) {
+                        bool clear_v, const float* sdf, int nblocks, uint2* list, uint32_t* count,
+                        uint32_t* next_count, hipStream_t s) {
     if (nblocks <= 0) return;
-    hipLaunchKernelGGL(k_set_u32_grid, dim3(1), dim3(1), 0, s, count, 0u);
+    if (!next_count) hipLaunchKernelGGL(k_set_u32_grid, dim3(1), dim3(1), 0, s, count, 0u);
     hipLaunchKernelGGL(k_grid_list, dim3((nblocks + 255) / 256), dim3(256), 0, s, cfgs, block_scene, m.flags,
                        v_clear.flags, clear_v ? 1 : 0, nblocks, list, count);
     const int waves = std::min((nblocks + 3) / 4, 8192);
     hipLaunchKernelGGL(k_grid_update, dim3((waves + 3) / 4), dim3(256), 0, s, cfgs, block_scene, m, v_clear, sdf,
-                       list, count, nscenes == 1 ? 1 : 0);
+                       list, count, next_count, nscenes == 1 ? 1 : 0, nblocks);
 }
 
 void launch_make_sdf(const DevCfg& c, const float* tris, int nt, float* sdf, hipStream_t s) {
