@@ -114,7 +114,7 @@ int mpm_engine_destroy(mpm_engine* h);
  *                   engine.cu:57].  The sort changes only the order of the
  *                   float atomics, not the exact-arithmetic result.
  *   "CHUNK"         maximum particles per transfer work item, a multiple of
- *                   64 in [64, 4096] [512]; changing it re-lays the particles
+ *                   64 in [64, 4096] [1024]; changing it re-lays the particles
  *   "LAUNCH_PCT"    transfer workgroups per 100 chunks, 1..100 [100]; below
  *                   100 each workgroup loops over several chunks
  *   "TIMING"        1 = record per-stage HIP events [0]
@@ -138,7 +138,8 @@ int mpm_engine_get_option(mpm_engine* h, const char* key, int64_t* value);
  * order); grids and SDFs are concatenated scene after scene
  * (mpm_batch_scene_info gives the offsets).  mpm_engine_set_sdf/get_sdf and
  * mpm_engine_get_grid address the concatenation.  Limits: nscenes <= 4096,
- * fewer than 2^31 grid nodes and 2^26 4^3-node blocks in all. */
+ * fewer than 2^31 grid nodes, 2^26 4^3-node blocks and 2^25 8x4x4-cell
+ * chunk blocks in all. */
 int mpm_batch_create(const mpm_sim_config* cfgs, int nscenes, mpm_engine** out);
 /* counts[s] particles per scene from a 26 x sum(counts) SoA buffer. */
 int mpm_batch_init_particles(mpm_engine* h, const int* counts, const float* soa26);

@@ -1,0 +1,88 @@
+"""GPU tests of the chunk layout built by the re-sort (no reference
+counterpart: the reference scatters particles in Morton order, engine.cu:54-70,
+without chunks or rounds).
+
+A chunk holds particles of one 8 x 4 x 4 block of base cells (trunc(x - 0.5)
+per axis, clamped into the grid).  With CHUNK large enough that no block is
+split, the chunk count is the number of non-empty blocks and a chunk has
+max(largest cell, ceil(count / 64)) rounds: the fewest that keep the base
+cells of each round distinct within 64 lanes.  Both are computed here with
+numpy from the initial positions.  Right after a re-sort no two lanes of a
+round share a base cell, so the first P2G needs no extra claim pass and no
+particle leaves its tile.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def expected_layout(cfg, pos, cap=None):
+    """(chunks, rounds) of one scene's particles; pos is 3 x n float32.
+    With cap, blocks split into ceil(count / cap) chunks and rounds is None
+    (the split of a block depends on atomic order)."""
+    size = np.array([cfg.SIZE_X, cfg.SIZE_Y, cfg.SIZE_Z], dtype=np.int64)[:, None]
+    base = np.trunc(pos.astype(np.float32) - np.float32(0.5)).astype(np.int64)
+    base = np.clip(base, 0, size - 1)
+    nbx, nby = (cfg.SIZE_X + 7) // 8, (cfg.SIZE_Y + 3) // 4
+    block = (base[0] >> 3) + nbx * ((base[1] >> 2) + nby * (base[2] >> 2))
+    cell = (base[0] & 7) + 8 * (base[1] & 3) + 32 * (base[2] & 3)
+    keys, cnt = np.unique(block * 128 + cell, return_counts=True)
+    blocks, inv = np.unique(keys // 128, return_inverse=True)
+    n_b = np.bincount(inv, weights=cnt).astype(np.int64)
+    if cap is not None:
+        return int(((n_b + cap - 1) // cap).sum()), None
+    max_b = np.zeros(len(blocks), dtype=np.int64)
+    np.maximum.at(max_b, inv, cnt)
+    return len(blocks), int(np.maximum(max_b, (n_b + 63) // 64).sum())
+
+
+def test_layout_single_scene(gpu):
+    sc = gpu.scenes.sphere64(physical=False)
+    p = sc["particles"]
+    e = gpu.Engine(sc["cfg"], CHUNK=4096)
+    e.init(p.shape[1], p)
+    e.set_sdf(sc["sdf"])
+    chunks, rounds = expected_layout(sc["cfg"], p[0:3])
+    got = e.counters()
+    say(f"sphere64 layout: n={p.shape[1]} chunks {got['chunks']} (expect {chunks}), rounds {got['rounds']} "
+        f"(expect {rounds}), lane use {p.shape[1] / (64 * rounds):.3f}")
+    assert got["chunks"] == chunks
+    assert got["rounds"] == rounds
+    e.step()
+    got = e.counters()
+    assert got["claim_losers"] == 0
+    assert got["tile_overflow_particles"] == 0
+    e.close()
+
+
+@pytest.mark.parametrize("cap", [64, 4096])
+def test_layout_odd_size_batch(gpu, cap):
+    """Scenes with an odd number of 4^3 blocks in x: their last block per
+    row is a single 4^3 block."""
+    b = gpu.scenes.batch_scenes(count=5, seed=23, n_min=2000, n_max=40000, edges=(21, 37, 45))
+    eb = gpu.Batch(b["cfgs"], CHUNK=cap)
+    eb.init(b["counts"], b["particles"])
+    for s, sdf in enumerate(b["sdfs"]):
+        eb.set_scene_sdf(s, sdf)
+    chunks = rounds = 0
+    lo = 0
+    for s, cfg in enumerate(b["cfgs"]):
+        hi = lo + int(b["counts"][s])
+        c, r = expected_layout(cfg, b["particles"][0:3, lo:hi], cap if cap < 4096 else None)
+        chunks += c
+        rounds += r or 0
+        lo = hi
+    got = eb.counters()
+    say(f"odd batch layout, CHUNK {cap}: chunks {got['chunks']} (expect {chunks}), rounds {got['rounds']}"
+        + ("" if cap < 4096 else f" (expect {rounds})"))
+    assert got["chunks"] == chunks
+    if cap == 4096:
+        assert got["rounds"] == rounds
+    eb.step()
+    assert eb.counters()["claim_losers"] == 0
+    eb.close()

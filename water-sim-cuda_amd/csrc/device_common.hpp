@@ -3,7 +3,7 @@
 // The per-particle and per-node arithmetic restates the reference's device
 // helpers (common.cuh:131-378) with the same operation order; the layout and
 // work decomposition are this engine's own (chunked Morton-sorted particles,
-// per-workgroup LDS tiles, 4^3 activity flags).
+// per-wave LDS tiles, 4^3 activity flags).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -66,37 +66,41 @@ enum : int {
     A_STATE = 24
 };
 
-// Chunk = up to CHUNK consecutive particles (in base-cell Morton order) whose
-// base cells (trunc(x - 0.5), the lowest node of the quadratic stencil) lie in
-// one 4^3 Morton block, processed by one transfer wave.  A chunk therefore
-// holds at most 64 groups of particles sharing a base cell.  It occupies the
-// slots [start, start + count) in rounds: round r holds the r-th member of
-// every group with more than r members (groups in key order), n_r <= 64
-// particles stored contiguously, with n_r in rounds[start + r].  The lanes of
-// a round hence hold particles with distinct base nodes and update distinct
-// arena nodes at each stencil offset.
+// Chunk = up to CHUNK consecutive particles (in base-cell order) whose base
+// cells (trunc(x - 0.5), the lowest node of the quadratic stencil) lie in one
+// 8 x 4 x 4 block, processed by one transfer wave.  It occupies the slots
+// [start, start + count) in S rounds; round r holds n_r <= 64 particles stored
+// contiguously, with n_r in rounds[start + r], and no two of them share a base
+// cell, so the lanes of a round update distinct arena nodes at each stencil
+// offset.  The layout is McNaughton's wrap-around rule: with the chunk's
+// particles listed cell after cell and S = max(largest cell, ceil(count/64)),
+// list entry q goes to lane q / S of round q % S.  A cell's entries are at
+// most S consecutive list positions, so they never meet in one round.
 struct ChunkDesc {
     int start;       // first slot (= first index of the chunk in sorted order)
     int count;       // particles in the chunk
     int ox, oy, oz;  // node origin of the chunk's LDS tile
-    int S;           // rounds (largest group)
+    int S;           // rounds
     int scene;       // scene of the chunk (index into the DevCfg table)
     int pad1;
 };
 
-constexpr int BLOCK_CELLS = 4;  // chunk block edge, base cells
-constexpr int BLOCK_SHIFT = 6;  // log2(4^3): Morton key -> block
-// Drift allowance of the tiles, in nodes.  Particles whose base node moved
+// Chunk block: 8 x 4 x 4 base cells, i.e. two x-adjacent 4^3 flag blocks
+// (the grid's activity flags keep their 4^3 granularity).  With 128 cells
+// per block, the rounds of a chunk can be filled close to 64 lanes even where
+// cell occupancies are uneven (see k_reorder).
+constexpr int BLOCK_X = 8, BLOCK_Y = 4, BLOCK_Z = 4;
+constexpr int BLOCK_BITS = 7;                    // log2(8 * 4 * 4): cell index bits of a bucket key
+constexpr int BLOCK_CELLS_N = 1 << BLOCK_BITS;   // 128
+// Nodes of a chunk's LDS tile per axis: the block's base cells plus the two
+// further nodes a quadratic stencil reaches.  Particles whose base node moved
 // out of their block since the last re-sort take the global-memory path.
-constexpr int TILE_MARGIN = 0;
-constexpr int TILE_D = BLOCK_CELLS + 2 + 2 * TILE_MARGIN;  // 6 nodes per axis
-constexpr int TILE_N = TILE_D * TILE_D * TILE_D;           // 216
+constexpr int TILE_DX = BLOCK_X + 2, TILE_DY = BLOCK_Y + 2, TILE_DZ = BLOCK_Z + 2;  // 10 x 6 x 6
+constexpr int TILE_N = TILE_DX * TILE_DY * TILE_DZ;                                // 360
 constexpr int TRANSFER_THREADS = 64;  // one wave per chunk (wave-private LDS arena)
 
-// LDS slot pitches of the tile/arena (16-byte slots).  Lanes of one LDS access
-// hold particles of neighbouring cells (a 2x2x2 group per 8 lanes); pitches
-// of 2 and 4 modulo 8 in y and z put such a group on 8 distinct 16-byte bank
-// groups.
+// LDS slot pitches of the tile/arena (16-byte slots), x fastest.  The
+// defaults store the 10 x 6 x 6 tile densely.
 #ifndef MPM_TILE_PY
 #define MPM_TILE_PY 10
 #endif
@@ -105,8 +109,15 @@ constexpr int TRANSFER_THREADS = 64;  // one wave per chunk (wave-private LDS ar
 #endif
 constexpr int TILE_PY = MPM_TILE_PY;
 constexpr int TILE_PZ = MPM_TILE_PZ;
-constexpr int TILE_SLOTS = (TILE_D - 1) * (1 + TILE_PY + TILE_PZ) + 1;
-static_assert(TILE_PY >= TILE_D && TILE_PZ >= TILE_PY * TILE_D, "tile pitches must not overlap");
+constexpr int TILE_SLOTS = (TILE_DX - 1) + TILE_PY * (TILE_DY - 1) + TILE_PZ * (TILE_DZ - 1) + 1;
+static_assert(TILE_PY >= TILE_DX && TILE_PZ >= TILE_PY * TILE_DY, "tile pitches must not overlap");
+
+// Tile coordinates of entry i of a dense x-fastest walk over the tile.
+__device__ __forceinline__ void tile_coords(int i, int& lx, int& ly, int& lz) {
+    lx = i % TILE_DX;
+    ly = (i / TILE_DX) % TILE_DY;
+    lz = i / (TILE_DX * TILE_DY);
+}
 
 // getGridIdx, common.cuh:131-134, offset to the scene's part of the grid.
 __device__ __forceinline__ int gidx(const DevCfg& c, int x, int y, int z) { return c.goff + z * c.sxy + y * c.sx + x; }

@@ -68,7 +68,7 @@ struct mpm_engine {
 
     int n = 0;
     bool have_particles = false;
-    int chunk_cap = 512;
+    int chunk_cap = 1024;
     int launch_pct = 100;  // transfer workgroups per 100 chunks
     int resort_every = 1;
     bool timing = false;
@@ -264,12 +264,13 @@ int ensure_planes(mpm_engine* h, int which, size_t slots) {
 }
 
 // Chunk-count bound for the current particles and chunk_cap: chunks are the
-// non-empty 4^3 blocks of every scene split into pieces of at most chunk_cap.
+// non-empty chunk blocks of every scene split into pieces of at most chunk_cap.
 int chunk_bound(const mpm_engine* h, int cap) {
     int64_t b = 0;
     for (int s = 0; s < h->S; ++s) {
         const int64_t ns = h->pstart[s + 1] - h->pstart[s];
-        b += std::min<int64_t>(ns, (int64_t)h->dcs[s].fbx * h->dcs[s].fby * h->dcs[s].fbz);
+        const DevCfg& d = h->dcs[s];
+        b += std::min<int64_t>(ns, (int64_t)((d.fbx + 1) / 2) * d.fby * d.fbz);
     }
     return (int)(b + (h->n + cap - 1) / cap + 1);
 }
@@ -528,38 +529,45 @@ int finish_steps(mpm_engine* h) {
     return MPM_OK;
 }
 
-// Numbers the flag blocks of every scene in Morton order of (bx, by, bz),
-// scene after scene, and allocates the re-sort buckets (cleared here; each
-// re-sort leaves them cleared).
+// Numbers the chunk blocks (pairs of x-adjacent 4^3 flag blocks; the last
+// one of a row is single when a scene has an odd number of flag blocks in x)
+// of every scene in Morton order of their coordinates, scene after scene:
+// rank[flag block] is its block's number, block[number] its lower flag block.
+// Allocates the re-sort buckets (cleared here; each re-sort leaves them
+// cleared).
 int build_block_table(mpm_engine* h) {
     auto spread = [](uint64_t v) {  // bit i of v -> bit 3i
         uint64_t r = 0;
         for (int i = 0; i < 21; ++i) r |= ((v >> i) & 1ull) << (3 * i);
         return r;
     };
-    const int nb = h->nfb;
-    std::vector<uint32_t> rank((size_t)nb), block((size_t)nb);
+    std::vector<uint32_t> rank((size_t)h->nfb), block;
     std::vector<std::pair<uint64_t, uint32_t>> order;
     uint32_t next = 0;
     for (const DevCfg& d : h->dcs) {
+        const int pbx = (d.fbx + 1) / 2;  // blocks per x row: pairs of flag blocks
         order.clear();
-        order.reserve((size_t)d.fbx * d.fby * d.fbz);
+        order.reserve((size_t)pbx * d.fby * d.fbz);
         for (int z = 0; z < d.fbz; ++z)
             for (int y = 0; y < d.fby; ++y)
-                for (int x = 0; x < d.fbx; ++x)
+                for (int x = 0; x < pbx; ++x)
                     order.emplace_back((spread((uint64_t)z) << 2) | (spread((uint64_t)y) << 1) | spread((uint64_t)x),
-                                       (uint32_t)(d.foff + x + d.fbx * (y + d.fby * z)));
+                                       (uint32_t)(d.foff + 2 * x + d.fbx * (y + d.fby * z)));
         std::sort(order.begin(), order.end());
         for (const auto& o : order) {
             rank[o.second] = next;
-            block[next] = o.second;
+            if ((int)(o.second - d.foff) % d.fbx + 1 < d.fbx) rank[o.second + 1] = next;
+            block.push_back(o.second);
             ++next;
         }
     }
+    const int nb = (int)next;
+    // bucket keys hold the block number above 7 cell bits
+    if (nb >= (1 << (32 - BLOCK_BITS))) return fail(MPM_ELIMIT, "scenes hold 2^25 or more 8x4x4 chunk blocks in all");
     BlockTable& b = h->blocks;
     b.nb = nb;
     int rc;
-    if ((rc = dev_alloc(&b.rank, (size_t)nb)) || (rc = dev_alloc(&b.block, (size_t)nb)) ||
+    if ((rc = dev_alloc(&b.rank, (size_t)h->nfb)) || (rc = dev_alloc(&b.block, (size_t)nb)) ||
         (rc = dev_alloc(&b.count, (size_t)nb)) || (rc = dev_alloc(&b.scan, (size_t)nb)) ||
         (rc = dev_alloc(&b.partial, scan_partial_elems((size_t)nb) + 16)))
         return rc;

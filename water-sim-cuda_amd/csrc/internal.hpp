@@ -102,12 +102,12 @@ struct SortScratch {
     size_t partial_cap;
 };
 
-// The flag blocks (4^3 nodes) of all scenes numbered densely, scene by scene
-// and within a scene in Morton order of (bx, by, bz), and the per-block
-// buckets of a re-sort.
+// The chunk blocks (8 x 4 x 4 base cells: pairs of x-adjacent 4^3 flag
+// blocks) of all scenes numbered densely, scene by scene and within a scene in
+// Morton order of their coordinates, and the per-block buckets of a re-sort.
 struct BlockTable {
-    uint32_t* rank = nullptr;     // nb: rank of each flag block
-    uint32_t* block = nullptr;    // nb: flag block of each rank
+    uint32_t* rank = nullptr;     // one per flag block: the number of its chunk block
+    uint32_t* block = nullptr;    // nb: lower flag block of each chunk block
     uint32_t* count = nullptr;    // nb: bucket sizes (all zero between re-sorts)
     uint64_t* scan = nullptr;     // nb: exclusive prefix of count | chunks << 32
     uint64_t* partial = nullptr;  // scan partials
@@ -115,7 +115,7 @@ struct BlockTable {
 };
 
 // One re-sort: the particles of (src, src_stride, src_orig) are grouped by
-// 4^3 block and laid out in rounds; map[new slot] = old slot, the caller
+// chunk block and laid out in rounds; map[new slot] = old slot, the caller
 // indices move to dst_orig and the planes in `mask` to (dst, dst_stride).
 struct ResortArgs {
     const float* src;

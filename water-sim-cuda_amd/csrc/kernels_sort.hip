@@ -3,11 +3,11 @@
 //
 // The reference sorts (key, iota) with thrust::sort_by_key every substep and
 // only uses the permutation as an index (engine.cu:92-102).  The engine's
-// re-sort instead moves the particle planes physically, into chunks (at most
-// CHUNK particles of one 4^3 block of base cells) whose particles sit in
-// rounds of distinct base cells:
+// re-sort instead lays particles out in chunks (at most CHUNK particles of one
+// 8 x 4 x 4 block of base cells) whose particles sit in rounds of distinct
+// base cells:
 //   1. bucket count: each particle's key = (dense Morton rank of its base
-//      cell's block) << 6 | (cell within the block); runs of equal blocks in
+//      cell's block) << 7 | (cell within the block); runs of equal blocks in
 //      a wave take their ranks in the bucket with one atomic;
 //   2. a scan over the blocks of (particles, chunks);
 //   3. a scatter into block order;
@@ -265,9 +265,10 @@ __global__ void k_keys_reference(const float* __restrict__ px, const float* __re
 // ---- the engine's re-sort ----
 __device__ __forceinline__ uint64_t lanemask_le(int lane) { return (2ull << lane) - 1ull; }
 
-// Bucket key of a position: the rank of its base cell's 4^3 block (base cells
-// clamped into the grid) above the 6-bit Morton index of the cell within the
-// block, so that 8 consecutive cell indices form a 2x2x2 group.
+// Bucket key of a position: the rank of its base cell's 8 x 4 x 4 block (base
+// cells clamped into the grid) above the 7-bit index of the cell within the
+// block: the Morton index within its 4^3 half, then the half.  Both 4^3 flag
+// blocks of a block carry the block's rank.
 __device__ __forceinline__ uint32_t bucket_key(const DevCfg& c, const uint32_t* __restrict__ brank, float x, float y,
                                                float z) {
     const int bx = min(max(base_of(x), 0), c.sx - 1);
@@ -275,8 +276,8 @@ __device__ __forceinline__ uint32_t bucket_key(const DevCfg& c, const uint32_t* 
     const int bz = min(max(base_of(z), 0), c.sz - 1);
     const int fb = c.foff + (bx >> 2) + c.fbx * ((by >> 2) + c.fby * (bz >> 2));
     const uint32_t cell = (uint32_t)((bx & 1) | ((by & 1) << 1) | ((bz & 1) << 2) | ((bx & 2) << 2) |
-                                     ((by & 2) << 3) | ((bz & 2) << 4));
-    return (brank[fb] << 6) | cell;
+                                     ((by & 2) << 3) | ((bz & 2) << 4) | ((bx & 4) << 4));
+    return (brank[fb] << BLOCK_BITS) | cell;
 }
 
 // 1. keys[i] and the particle's place within its bucket (rank[i]); bcount
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(256) void k_bucket_count(const float* __restrict__ 
         if (valid) {
             const int sc = t.nscenes > 1 ? scene_of(t, orig ? orig[i] : i) : 0;
             key[j] = bucket_key(t.cfgs[sc], brank, px[i], py[i], pz[i]);
-            b[j] = key[j] >> 6;
+            b[j] = key[j] >> BLOCK_BITS;
         }
         const uint32_t prev = __shfl_up(b[j], 1);
         const bool head = valid && (lane == 0 || prev != b[j]);
@@ -343,15 +344,16 @@ __global__ __launch_bounds__(256) void k_bucket_scatter(const uint32_t* __restri
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t key = keys[i];
-    const uint32_t pos = (uint32_t)bscan[key >> 6] + rank[i];
+    const uint32_t pos = (uint32_t)bscan[key >> BLOCK_BITS] + rank[i];
     kout[pos] = key;
     vout[pos] = (uint32_t)i;
 }
 
-// 4. Chunks of every non-empty bucket (pieces of at most cap particles), and
-// the bucket cleared for the next re-sort.  Thread 0 publishes the chunk count
-// (at most `limit`, which bounds it by construction) and clears the round
-// total.
+// 4. Chunks of every non-empty bucket: ceil(size / cap) pieces of equal size
+// (within one particle), and the bucket cleared for the next re-sort.
+// Thread 0 publishes the chunk count (at most `limit`, which bounds it by
+// construction) and clears the round total.  bblock holds the lower (even x)
+// flag block of each block.
 __global__ __launch_bounds__(256) void k_bucket_chunks(uint32_t* __restrict__ bcount,
                                                        const uint64_t* __restrict__ bscan,
                                                        const uint32_t* __restrict__ bblock,
@@ -378,14 +380,17 @@ __global__ __launch_bounds__(256) void k_bucket_chunks(uint32_t* __restrict__ bc
     const int local = fb - cf.foff;
     const int bx = local % cf.fbx, byz = local / cf.fbx;
     const int by = byz % cf.fby, bz = byz / cf.fby;
-    for (int off = 0; off < (int)cnt; off += cap, ++c) {
+    const int pieces = (int)((cnt + (uint32_t)cap - 1u) / (uint32_t)cap);
+    for (int k = 0; k < pieces; ++k, ++c) {
         if (c >= limit) return;
+        const int lo = (int)((uint64_t)k * cnt / (uint32_t)pieces);
+        const int hi = (int)((uint64_t)(k + 1) * cnt / (uint32_t)pieces);
         ChunkDesc d;
-        d.start = start + off;
-        d.count = min(cap, (int)cnt - off);
-        d.ox = 4 * bx - TILE_MARGIN;
-        d.oy = 4 * by - TILE_MARGIN;
-        d.oz = 4 * bz - TILE_MARGIN;
+        d.start = start + lo;
+        d.count = hi - lo;
+        d.ox = 4 * bx;
+        d.oy = 4 * by;
+        d.oz = 4 * bz;
         d.S = 0;  // set by the reorder
         d.scene = scene;
         d.pad1 = 0;
@@ -396,13 +401,15 @@ __global__ __launch_bounds__(256) void k_bucket_chunks(uint32_t* __restrict__ bc
 constexpr int RO_THREADS = 256;
 
 // 5. One workgroup per chunk (grid-stride over the chunk table).  In LDS: the
-// chunk's particles counted and ranked by cell (at most 64 cells: one 4^3
-// block), their old slots put in cell order, and the rounds laid out -- round
-// r takes the r-th member of every cell with more than r -- so the lanes of a
-// round hold distinct base cells.  Then map[new slot] = old slot, and the
-// caller indices and the planes in `mask` (bit a = attribute a) move to the
-// new slots (PLANES = false: mask is 0, only the map and the caller indices;
-// fewer registers, more chunks in flight).  Dynamic LDS: 3 x cap words.
+// chunk's particles counted and ranked by cell (at most 128 cells: one
+// 8 x 4 x 4 block), their old slots listed cell after cell, and the rounds
+// laid out by the wrap-around rule (ChunkDesc): with S = max(largest cell,
+// ceil(count / 64)) rounds, list entry q goes to lane q / S of round q % S;
+// rounds 0 .. full-1 hold nl = ceil(count / S) lanes and the others nl - 1.
+// Then map[new slot] = old slot, and the caller indices and the planes in
+// `mask` (bit a = attribute a) move to the new slots (PLANES = false: mask is
+// 0, only the map and the caller indices; fewer registers, more chunks in
+// flight).  Dynamic LDS: 3 x cap words.
 template <bool PLANES>
 __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ ch,
                                                         const uint32_t* __restrict__ counters, int cap,
@@ -414,64 +421,67 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
                                                         uint8_t* __restrict__ rounds,
                                                         uint32_t* __restrict__ rounds_total,
                                                         uint32_t* __restrict__ map, uint32_t mask) {
+    static_assert(BLOCK_CELLS_N == 128 && RO_THREADS >= 128, "two waves scan the cell counts");
     extern __shared__ uint32_t ro_lds[];
-    uint32_t* cell_rank = ro_lds;       // [cap] cell | rank << 6 of each chunk position
-    uint32_t* by_cell = ro_lds + cap;   // [cap] old slots in cell order
+    uint32_t* cell_rank = ro_lds;          // [cap] cell | rank << BLOCK_BITS of each chunk position
+    uint32_t* by_cell = ro_lds + cap;      // [cap] old slots, cell after cell
     uint32_t* slot_of = ro_lds + 2 * cap;  // [cap] old slots of the chunk, then of each new slot
-    __shared__ uint32_t cnt[64];
-    __shared__ uint32_t cfirst[64];
-    const int tid = threadIdx.x, lane = tid & 63;
+    __shared__ uint32_t cnt[BLOCK_CELLS_N];
+    __shared__ uint32_t cfirst[BLOCK_CELLS_N];
+    __shared__ uint32_t wtot[2], wmax[2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t nch = counters[3];
     uint32_t rsum = 0u;
     for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
         const ChunkDesc d = uniform(ch[ci]);
-        if (tid < 64) cnt[tid] = 0u;
+        if (tid < BLOCK_CELLS_N) cnt[tid] = 0u;
         __syncthreads();
 #pragma unroll 4
         for (int q = tid; q < d.count; q += RO_THREADS) {
-            cell_rank[q] = keys[d.start + q] & 63u;
+            cell_rank[q] = keys[d.start + q] & (uint32_t)(BLOCK_CELLS_N - 1);
             slot_of[q] = vals[d.start + q];
         }
         __syncthreads();
         for (int q = tid; q < d.count; q += RO_THREADS) {
             const uint32_t c = cell_rank[q];
-            cell_rank[q] = c | (atomicAdd(&cnt[c], 1u) << 6);
+            cell_rank[q] = c | (atomicAdd(&cnt[c], 1u) << BLOCK_BITS);
         }
         __syncthreads();
-        if (tid < 64) {
-            const uint32_t size = cnt[lane];
-            uint32_t x = size;
+        if (tid < BLOCK_CELLS_N) {  // waves 0 and 1: 64 cells each
+            const uint32_t size = cnt[tid];
+            uint32_t x = size, m = size;
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
                 const uint32_t y = __shfl_up(x, o);
                 if (lane >= o) x += y;
+                m = max(m, (uint32_t)__shfl_xor((int)m, o));
             }
-            cfirst[lane] = x - size;
+            cfirst[tid] = x - size;  // within the wave's 64 cells
+            if (lane == 63) {
+                wtot[wave] = x;
+                wmax[wave] = m;
+            }
         }
         __syncthreads();
+        const uint32_t n = (uint32_t)d.count;
+        const uint32_t S = max(max(wmax[0], wmax[1]), (n + 63u) / 64u);
+        const uint32_t nl = (n + S - 1u) / S;     // lanes of the first rounds (<= 64 as S >= n / 64)
+        const uint32_t full = n - (nl - 1u) * S;  // rounds with nl lanes, 1 .. S
         for (int q = tid; q < d.count; q += RO_THREADS) {
-            const uint32_t v = cell_rank[q];
-            by_cell[cfirst[v & 63u] + (v >> 6)] = slot_of[q];
+            const uint32_t v = cell_rank[q], c = v & (uint32_t)(BLOCK_CELLS_N - 1);
+            by_cell[cfirst[c] + (c >= 64u ? wtot[0] : 0u) + (v >> BLOCK_BITS)] = slot_of[q];
         }
         __syncthreads();
-        if (tid < 64) {
-            const uint32_t size = cnt[lane], first = cfirst[lane];
-            uint32_t S = size;
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) S = max(S, (uint32_t)__shfl_xor((int)S, o));
-            S = __builtin_amdgcn_readfirstlane(S);
-            uint32_t off = 0u;
-            for (uint32_t r = 0; r < S; ++r) {
-                const bool has = size > r;
-                const uint64_t m = __ballot(has);
-                if (has) slot_of[off + (uint32_t)__popcll(m & lanemask_lt(lane))] = by_cell[first + r];
-                if (lane == 0) rounds[d.start + r] = (uint8_t)__popcll(m);
-                off += (uint32_t)__popcll(m);
-            }
-            if (lane == 0) {
-                ch[ci].S = (int)S;
-                rsum += S;
-            }
+        for (uint32_t q = (uint32_t)tid; q < n; q += RO_THREADS) {
+            const uint32_t L = q / S, r = q - L * S;
+            const uint32_t o = (r < full ? r * nl : full * nl + (r - full) * (nl - 1u)) + L;
+            slot_of[o] = by_cell[q];
+        }
+        for (uint32_t r = (uint32_t)tid; r < S; r += RO_THREADS)
+            rounds[d.start + r] = (uint8_t)(r < full ? nl : nl - 1u);
+        if (tid == 0) {
+            ch[ci].S = (int)S;
+            rsum += S;
         }
         __syncthreads();
         for (int q = tid; q < d.count; q += RO_THREADS) {

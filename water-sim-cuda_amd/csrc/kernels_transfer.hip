@@ -1,13 +1,13 @@
 // kernels_transfer.hip -- particle<->grid transfer kernels.
 //
-// One single-wave workgroup per chunk (<= CHUNK particles of one 4^3-cell
-// Morton block).  The wave stages the grid velocity of its block's
+// One single-wave workgroup per chunk (<= CHUNK particles of one 8 x 4 x 4
+// block of base cells).  The wave stages the grid velocity of its block's
 // neighbourhood in an LDS tile (G2P reads) and accumulates its P2G
 // contributions in an LDS arena of the same footprint, which it flushes with
 // one global float atomic per node and channel.  Particles whose stencil
-// leaves the tile (they drifted more than TILE_MARGIN nodes out of their
-// block since the last re-sort, or sit far outside the grid) fall back to
-// global loads/atomics.
+// leaves the tile (their base cell moved out of their block since the last
+// re-sort, or they sit far outside the grid) fall back to global
+// loads/atomics.
 //
 // Arena accumulation.  FP32 LDS atomics (ds_add_f32) run at about a third of
 // a lane per clock per CU on gfx950, some 40x below integer LDS atomics, so
@@ -117,8 +117,8 @@ __device__ __forceinline__ float sdf_sample(const DevCfg& c, const float* __rest
                                             int oy, int oz, float px, float py, float pz) {
     const int i = (int)floorf(px), j = (int)floorf(py), k = (int)floorf(pz);
     const int lx = i - ox, ly = j - oy, lz = k - oz;
-    if ((unsigned)lx <= (unsigned)(TILE_D - 2) && (unsigned)ly <= (unsigned)(TILE_D - 2) &&
-        (unsigned)lz <= (unsigned)(TILE_D - 2) && i >= 0 && j >= 0 && k >= 0 && i + 1 < c.sx && j + 1 < c.sy &&
+    if ((unsigned)lx <= (unsigned)(TILE_DX - 2) && (unsigned)ly <= (unsigned)(TILE_DY - 2) &&
+        (unsigned)lz <= (unsigned)(TILE_DZ - 2) && i >= 0 && j >= 0 && k >= 0 && i + 1 < c.sx && j + 1 < c.sy &&
         k + 1 < c.sz) {
         const float4* b = vt + (lx + TILE_PY * ly + TILE_PZ * lz);
         return trilerp(b[0].w, b[1].w, b[TILE_PY].w, b[TILE_PY + 1].w, b[TILE_PZ].w, b[TILE_PZ + 1].w,
@@ -268,8 +268,8 @@ __device__ __forceinline__ void load_particle(const float* __restrict__ part, si
 }
 
 __device__ __forceinline__ bool stencil_in_tile(int lx, int ly, int lz) {
-    return (unsigned)lx <= (unsigned)(TILE_D - 3) && (unsigned)ly <= (unsigned)(TILE_D - 3) &&
-           (unsigned)lz <= (unsigned)(TILE_D - 3);
+    return (unsigned)lx <= (unsigned)(TILE_DX - 3) && (unsigned)ly <= (unsigned)(TILE_DY - 3) &&
+           (unsigned)lz <= (unsigned)(TILE_DZ - 3);
 }
 
 // GATHER: the particles are read through a.map from a.src (the first G2P
@@ -300,7 +300,8 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
     const int lane = threadIdx.x;
     if constexpr (DO_G2P) {
         for (int i = lane; i < TILE_N; i += TRANSFER_THREADS) {
-            int lx = i % TILE_D, ly = (i / TILE_D) % TILE_D, lz = i / (TILE_D * TILE_D);
+            int lx, ly, lz;
+            tile_coords(i, lx, ly, lz);
             int x = d.ox + lx, y = d.oy + ly, z = d.oz + lz;
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
             if (inb(c, x, y, z)) {
@@ -464,7 +465,8 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
     if constexpr (DO_P2G) {
         __syncthreads();
         for (int i = lane; i < TILE_N; i += TRANSFER_THREADS) {
-            const int lx = i % TILE_D, ly = (i / TILE_D) % TILE_D, lz = i / (TILE_D * TILE_D);
+            int lx, ly, lz;
+            tile_coords(i, lx, ly, lz);
             const float4 v = ar[lx + TILE_PY * ly + TILE_PZ * lz];
             if (v.x > 1e-9f) {  // the reference's tile flush threshold, kernels.cu:132
                 int x = d.ox + lx, y = d.oy + ly, z = d.oz + lz;
