@@ -114,10 +114,11 @@ int mpm_engine_destroy(mpm_engine* h);
  *                   engine.cu:57].  The sort changes only the order of the
  *                   float atomics, not the exact-arithmetic result.
  *   "CHUNK"         maximum particles per transfer work item, a multiple of
- *                   64 in [64, 4096] [1024]; changing it re-lays the particles
+ *                   64 in [64, 2048] [1024]; changing it re-lays the particles
  *   "LAUNCH_PCT"    transfer workgroups per 100 chunks, 1..100 [100]; below
  *                   100 each workgroup loops over several chunks
- *   "TIMING"        1 = record per-stage HIP events [0]
+ *   "TIMING"        1 = record per-stage HIP events, 2 = around the transfer
+ *                   stages only (fewer events in the stream) [0]
  *   "GRAPH"         1 = capture a steps(k) call as a HIP graph once and replay
  *                   it while the schedule state matches [0]
  *   "SDF_BVH"       1 = initSDF through a bounding-volume hierarchy [1]: the
@@ -219,7 +220,7 @@ int mpm_engine_get_sort(mpm_engine* h, int32_t* keys_out, int32_t* perm_out);
  * velocity after gridUpdate, kernels.cu:193). */
 int mpm_engine_get_grid(mpm_engine* h, float* grid4_out);
 
-/* Per-stage device time accumulated while TIMING=1, in milliseconds, and the
+/* Per-stage device time accumulated while TIMING is on, in milliseconds, and the
  * number of launches: out[2*s] = ms, out[2*s+1] = count for stage s in
  * 0 = P2G, 1 = G2P, 2 = fused G2P+P2G, 3 = grid update, 4 = re-sort.
  * Resets the accumulators. */

@@ -44,7 +44,7 @@ def expected_layout(cfg, pos, cap=None):
 def test_layout_single_scene(gpu):
     sc = gpu.scenes.sphere64(physical=False)
     p = sc["particles"]
-    e = gpu.Engine(sc["cfg"], CHUNK=4096)
+    e = gpu.Engine(sc["cfg"], CHUNK=2048)
     e.init(p.shape[1], p)
     e.set_sdf(sc["sdf"])
     chunks, rounds = expected_layout(sc["cfg"], p[0:3])
@@ -60,7 +60,7 @@ def test_layout_single_scene(gpu):
     e.close()
 
 
-@pytest.mark.parametrize("cap", [64, 4096])
+@pytest.mark.parametrize("cap", [64, 2048])
 def test_layout_odd_size_batch(gpu, cap):
     """Scenes with an odd number of 4^3 blocks in x: their last block per
     row is a single 4^3 block."""
@@ -73,15 +73,15 @@ def test_layout_odd_size_batch(gpu, cap):
     lo = 0
     for s, cfg in enumerate(b["cfgs"]):
         hi = lo + int(b["counts"][s])
-        c, r = expected_layout(cfg, b["particles"][0:3, lo:hi], cap if cap < 4096 else None)
+        c, r = expected_layout(cfg, b["particles"][0:3, lo:hi], cap if cap < 2048 else None)
         chunks += c
         rounds += r or 0
         lo = hi
     got = eb.counters()
     say(f"odd batch layout, CHUNK {cap}: chunks {got['chunks']} (expect {chunks}), rounds {got['rounds']}"
-        + ("" if cap < 4096 else f" (expect {rounds})"))
+        + ("" if cap < 2048 else f" (expect {rounds})"))
     assert got["chunks"] == chunks
-    if cap == 4096:
+    if cap == 2048:
         assert got["rounds"] == rounds
     eb.step()
     assert eb.counters()["claim_losers"] == 0

@@ -71,7 +71,7 @@ struct mpm_engine {
     int chunk_cap = 1024;
     int launch_pct = 100;  // transfer workgroups per 100 chunks
     int resort_every = 1;
-    bool timing = false;
+    int timing = 0;  // 0 off, 1 every stage, 2 the transfer stages only
     bool use_graph = false;
     bool capturing = false;
     bool sdf_bvh = true;             // makeSDF through the hierarchy (same field)
@@ -174,7 +174,7 @@ struct StageTimer {
     int stage;
     hipEvent_t a = nullptr;
     StageTimer(mpm_engine* h_, int s) : h(h_), stage(s) {
-        if (h->timing && !h->capturing) {
+        if (h->timing && !h->capturing && (h->timing == 1 || s <= ST_G2P2G)) {
             a = take_event(h);
             hipEventRecord(a, h->stream);
         }
@@ -343,7 +343,9 @@ int sort_and_reorder(mpm_engine* h, const float* src, size_t src_stride, const i
 constexpr uint32_t MASK_ALL = (1u << A_STATE) - 1u;
 
 // full: every plane moves now.  Otherwise (inside steps(), where a G2P always
-// follows) no plane moves: that G2P reads positions and F through the map.
+// follows) no plane moves: until that G2P has written positions and F to the
+// new planes, the transfers read the old planes through the map (a P2G all
+// attributes, the G2P positions and F; v and C then come from the G2P).
 int resort(mpm_engine* h, bool full) {
     StageTimer t(h, ST_SORT);
     const int src = h->pcur;
@@ -386,7 +388,8 @@ TransferArgs make_args(mpm_engine* h) {
 void run_transfer(mpm_engine* h, int mode, int launch) {
     const int stage = mode == XFER_P2G ? ST_P2G : (mode == XFER_G2P ? ST_G2P : ST_G2P2G);
     StageTimer t(h, stage);
-    // a pending gather is always consumed by a G2P (resort() keeps it so)
+    // a pending gather is always consumed by a G2P (resort() keeps it so); a
+    // P2G before it reads through the map too
     launch_transfer(mode, make_args(h), launch, h->stream);
     if (mode & XFER_G2P) h->gather_pending = false;
 }
@@ -443,8 +446,11 @@ int enqueue_steps(mpm_engine* h, int k, int launch) {
     launch_set_u32(h->overflow, 1, 0u, h->stream);
     for (int s = 0; s < k; ++s) {
         if (s == 0) {
-            if (h->substeps - h->sorted_at >= h->resort_every) {
-                if ((rc = resort(h, true))) return rc;
+            // A re-sort due now waits for the first fused substep when there
+            // is one (reading all attributes through the map would cost the
+            // P2G more than one substep's drift).
+            if (k == 1 && h->substeps - h->sorted_at >= h->resort_every) {
+                if ((rc = resort(h, false))) return rc;
                 h->sorted_at = h->substeps;
             }
             run_transfer(h, XFER_P2G, launch);
@@ -834,8 +840,8 @@ int mpm_engine_set_option(mpm_engine* h, const char* key, int64_t value) {
         if (value < 1) return fail(MPM_EINVAL, "RESORT_EVERY must be >= 1");
         h->resort_every = (int)std::min<int64_t>(value, 1 << 30);
     } else if (k == "CHUNK") {
-        if (value < 64 || value > 4096 || (value % 64) != 0)
-            return fail(MPM_EINVAL, "CHUNK must be a multiple of 64 in [64, 4096]");
+        if (value < 64 || value > 2048 || (value % 64) != 0)
+            return fail(MPM_EINVAL, "CHUNK must be a multiple of 64 in [64, 2048]");
         if (value == h->chunk_cap) return MPM_OK;
         if (!h->have_particles || h->n == 0) {
             h->chunk_cap = (int)value;
@@ -856,7 +862,8 @@ int mpm_engine_set_option(mpm_engine* h, const char* key, int64_t value) {
     } else if (k == "SDF_BVH") {
         h->sdf_bvh = value != 0;
     } else if (k == "TIMING") {
-        h->timing = value != 0;
+        if (value < 0 || value > 2) return fail(MPM_EINVAL, "TIMING must be 0, 1 or 2");
+        h->timing = (int)value;
     } else if (k == "GRAPH") {
         h->use_graph = value != 0;
         if (!h->use_graph) drop_graph(h);
@@ -876,7 +883,7 @@ int mpm_engine_get_option(mpm_engine* h, const char* key, int64_t* value) {
     if (k == "RESORT_EVERY") *value = h->resort_every;
     else if (k == "CHUNK") *value = h->chunk_cap;
     else if (k == "LAUNCH_PCT") *value = h->launch_pct;
-    else if (k == "TIMING") *value = h->timing ? 1 : 0;
+    else if (k == "TIMING") *value = h->timing;
     else if (k == "SDF_BVH") *value = h->sdf_bvh ? 1 : 0;
     else if (k == "SDF_EXACT_NODES") *value = h->sdf_exact_nodes;
     else if (k == "GRAPH") *value = h->use_graph ? 1 : 0;

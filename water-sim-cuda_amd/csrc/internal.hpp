@@ -36,9 +36,9 @@ struct TransferArgs {
     uint8_t* acc_flags;
     const float* sdf;
     unsigned int* overflow;  // [0] particles outside their tile, [1] particles that lost a base-node claim
-    // Non-null for the first transfer after a re-sort that moved no planes
-    // (modes with G2P only): slot s reads its position and F from `src` at
-    // old slot map[s]; the kernel writes the particle to `part` as usual.
+    // Non-null for the transfers after a re-sort that moved no planes, up to
+    // the first one with G2P: slot s reads its particle from `src` at old
+    // slot map[s]; a G2P writes it to `part` as usual.
     const uint32_t* map;
     const float* src;
     size_t src_stride;

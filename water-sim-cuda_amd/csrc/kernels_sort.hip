@@ -402,14 +402,24 @@ constexpr int RO_THREADS = 256;
 
 // 5. One workgroup per chunk (grid-stride over the chunk table).  In LDS: the
 // chunk's particles counted and ranked by cell (at most 128 cells: one
-// 8 x 4 x 4 block), their old slots listed cell after cell, and the rounds
-// laid out by the wrap-around rule (ChunkDesc): with S = max(largest cell,
-// ceil(count / 64)) rounds, list entry q goes to lane q / S of round q % S;
-// rounds 0 .. full-1 hold nl = ceil(count / S) lanes and the others nl - 1.
-// Then map[new slot] = old slot, and the caller indices and the planes in
-// `mask` (bit a = attribute a) move to the new slots (PLANES = false: mask is
-// 0, only the map and the caller indices; fewer registers, more chunks in
-// flight).  Dynamic LDS: 3 x cap words.
+// 8 x 4 x 4 block) and listed cell after cell, the members of a cell in
+// caller-index order, and the rounds laid out by the wrap-around rule
+// (ChunkDesc): with S = max(largest cell, ceil(count / 64)) rounds, list entry
+// q goes to lane q / S of round q % S; rounds 0 .. full-1 hold
+// nl = ceil(count / S) lanes and the others nl - 1.  Then map[new slot] = old
+// slot, and the caller indices and the planes in `mask` (bit a = attribute a)
+// move to the new slots (PLANES = false: mask is 0, only the map and the
+// caller indices; fewer registers, more chunks in flight).
+//
+// Ordering a cell's members by caller index (not by the order of the LDS
+// atomics) makes the layout a function of the particles' cells alone: a
+// particle that stayed in its cell keeps its list position up to the shifts
+// caused by particles that moved, so the map is close to the identity and the
+// next transfer's reads through it stay coalesced.  Cells with more than
+// RO_SORT_MAX members (far denser than a fluid) keep the atomics' order.
+// Dynamic LDS: 4 x cap words (cap <= 2048 as chunk positions take 16 bits
+// in the list entries and 64 KB bound the workgroup's LDS).
+constexpr int RO_SORT_MAX = 64;
 template <bool PLANES>
 __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ ch,
                                                         const uint32_t* __restrict__ counters, int cap,
@@ -423,9 +433,12 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
                                                         uint32_t* __restrict__ map, uint32_t mask) {
     static_assert(BLOCK_CELLS_N == 128 && RO_THREADS >= 128, "two waves scan the cell counts");
     extern __shared__ uint32_t ro_lds[];
-    uint32_t* cell_rank = ro_lds;          // [cap] cell | rank << BLOCK_BITS of each chunk position
-    uint32_t* by_cell = ro_lds + cap;      // [cap] old slots, cell after cell
-    uint32_t* slot_of = ro_lds + 2 * cap;  // [cap] old slots of the chunk, then of each new slot
+    uint32_t* cell_rank = ro_lds;          // [cap] cell | atomic rank << BLOCK_BITS per chunk position;
+                                           //       then the list in final order (chunk positions)
+    uint32_t* slot_of = ro_lds + cap;      // [cap] old slot per chunk position
+    uint32_t* org = ro_lds + 2 * cap;      // [cap] caller index per chunk position
+    uint32_t* lst = ro_lds + 3 * cap;      // [cap] list in atomic order (position | cell << 16);
+                                           //       then the chunk position of each new slot
     __shared__ uint32_t cnt[BLOCK_CELLS_N];
     __shared__ uint32_t cfirst[BLOCK_CELLS_N];
     __shared__ uint32_t wtot[2], wmax[2];
@@ -438,8 +451,10 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
         __syncthreads();
 #pragma unroll 4
         for (int q = tid; q < d.count; q += RO_THREADS) {
+            const uint32_t si = vals[d.start + q];
             cell_rank[q] = keys[d.start + q] & (uint32_t)(BLOCK_CELLS_N - 1);
-            slot_of[q] = vals[d.start + q];
+            slot_of[q] = si;
+            org[q] = src_orig ? (uint32_t)src_orig[si] : si;
         }
         __syncthreads();
         for (int q = tid; q < d.count; q += RO_THREADS) {
@@ -463,19 +478,33 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
             }
         }
         __syncthreads();
+        if (tid >= 64 && tid < BLOCK_CELLS_N) cfirst[tid] += wtot[0];  // wave 1's cells follow wave 0's
+        __syncthreads();
+        for (int q = tid; q < d.count; q += RO_THREADS) {
+            const uint32_t v = cell_rank[q], c = v & (uint32_t)(BLOCK_CELLS_N - 1);
+            lst[cfirst[c] + (v >> BLOCK_BITS)] = (uint32_t)q | (c << 16);
+        }
+        __syncthreads();
+        // each list entry's rank among its cell's members by caller index
+        for (int p = tid; p < d.count; p += RO_THREADS) {
+            const uint32_t e = lst[p], q = e & 0xFFFFu, c = e >> 16;
+            const uint32_t first = cfirst[c], size = cnt[c];
+            uint32_t rank = (uint32_t)p - first;
+            if (size <= (uint32_t)RO_SORT_MAX) {
+                const uint32_t g = org[q];
+                rank = 0u;
+                for (uint32_t k = 0; k < size; ++k) rank += org[lst[first + k] & 0xFFFFu] < g ? 1u : 0u;
+            }
+            cell_rank[first + rank] = q;
+        }
+        __syncthreads();
         const uint32_t n = (uint32_t)d.count;
         const uint32_t S = max(max(wmax[0], wmax[1]), (n + 63u) / 64u);
         const uint32_t nl = (n + S - 1u) / S;     // lanes of the first rounds (<= 64 as S >= n / 64)
         const uint32_t full = n - (nl - 1u) * S;  // rounds with nl lanes, 1 .. S
-        for (int q = tid; q < d.count; q += RO_THREADS) {
-            const uint32_t v = cell_rank[q], c = v & (uint32_t)(BLOCK_CELLS_N - 1);
-            by_cell[cfirst[c] + (c >= 64u ? wtot[0] : 0u) + (v >> BLOCK_BITS)] = slot_of[q];
-        }
-        __syncthreads();
         for (uint32_t q = (uint32_t)tid; q < n; q += RO_THREADS) {
             const uint32_t L = q / S, r = q - L * S;
-            const uint32_t o = (r < full ? r * nl : full * nl + (r - full) * (nl - 1u)) + L;
-            slot_of[o] = by_cell[q];
+            lst[(r < full ? r * nl : full * nl + (r - full) * (nl - 1u)) + L] = cell_rank[q];
         }
         for (uint32_t r = (uint32_t)tid; r < S; r += RO_THREADS)
             rounds[d.start + r] = (uint8_t)(r < full ? nl : nl - 1u);
@@ -484,10 +513,11 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
             rsum += S;
         }
         __syncthreads();
-        for (int q = tid; q < d.count; q += RO_THREADS) {
+        for (int o = tid; o < d.count; o += RO_THREADS) {
+            const uint32_t q = lst[o];
             const uint32_t si = slot_of[q];
-            const size_t o = (size_t)d.start + q;
-            map[o] = si;
+            const size_t t = (size_t)d.start + o;
+            map[t] = si;
             if constexpr (PLANES) {
                 float v[A_STATE];
 #pragma unroll
@@ -495,9 +525,9 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
                     if (mask & (1u << a)) v[a] = src[(size_t)a * src_stride + si];
 #pragma unroll
                 for (int a = 0; a < A_STATE; ++a)
-                    if (mask & (1u << a)) dst[(size_t)a * dst_stride + o] = v[a];
+                    if (mask & (1u << a)) dst[(size_t)a * dst_stride + t] = v[a];
             }
-            dst_orig[o] = src_orig ? src_orig[si] : (int)si;
+            dst_orig[t] = (int)org[q];
         }
         __syncthreads();  // the LDS arrays are reused by the next chunk
     }
@@ -576,7 +606,7 @@ void launch_resort(const ResortArgs& a, hipStream_t s) {
                        a.chunk_limit);
     if (n <= 0) return;
     const int grid = std::max(1, std::min(a.chunk_est, 16384));
-    const size_t lds = (size_t)3 * a.chunk_cap * sizeof(uint32_t);
+    const size_t lds = (size_t)4 * a.chunk_cap * sizeof(uint32_t);
     if (a.mask)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_reorder<true>), dim3((unsigned)grid), dim3(RO_THREADS), lds, s, a.chunks,
                            a.counters, a.chunk_cap, a.keys[1], a.vals[1], a.src, a.src_stride, a.src_orig, a.dst,

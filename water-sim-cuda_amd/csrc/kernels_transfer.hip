@@ -272,8 +272,8 @@ __device__ __forceinline__ bool stencil_in_tile(int lx, int ly, int lz) {
            (unsigned)lz <= (unsigned)(TILE_DZ - 3);
 }
 
-// GATHER: the particles are read through a.map from a.src (the first G2P
-// after a re-sort that left the planes in place).
+// GATHER: the particles are read through a.map from a.src (the transfers
+// after a re-sort that left the planes in place, up to the first G2P).
 template <int MODE, bool GATHER>
 // At most 168 VGPRs: three waves per SIMD (LDS allows about 13 chunks per CU).
 __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a) {
@@ -490,9 +490,12 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
 void launch_transfer(int mode, const TransferArgs& a, int grid_chunks, hipStream_t s) {
     const int per = (grid_chunks + 7) / 8;
     dim3 grid(8 * (per < 1 ? 1 : per)), block(TRANSFER_THREADS);
-    const bool g = a.map != nullptr;  // the engine passes a map only with G2P modes
+    const bool g = a.map != nullptr;  // particles read through the map from a.src
     switch (mode) {
-        case XFER_P2G: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_transfer<XFER_P2G, false>), grid, block, 0, s, a); break;
+        case XFER_P2G:
+            if (g) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_transfer<XFER_P2G, true>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_transfer<XFER_P2G, false>), grid, block, 0, s, a);
+            break;
         case XFER_G2P:
             if (g) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_transfer<XFER_G2P, true>), grid, block, 0, s, a);
             else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_transfer<XFER_G2P, false>), grid, block, 0, s, a);

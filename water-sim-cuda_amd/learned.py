@@ -174,7 +174,7 @@ def rms(a: np.ndarray, b: np.ndarray) -> float:
 # on the dam-break family (MI355X): the plain step at DT, and the corrected
 # or uncorrected step at RATIO*DT (particles move RATIO times farther per
 # substep, so they need re-sorting sooner).
-PLAIN_RESORT = 5
+PLAIN_RESORT = 4
 COARSE_RESORT = 2
 
 
