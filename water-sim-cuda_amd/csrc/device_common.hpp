@@ -70,12 +70,13 @@ enum : int {
 // cells (trunc(x - 0.5), the lowest node of the quadratic stencil) lie in one
 // 8 x 4 x 4 block, processed by one transfer wave.  It occupies the slots
 // [start, start + count) in S rounds; round r holds n_r <= 64 particles stored
-// contiguously, with n_r in rounds[start + r], and no two of them share a base
-// cell, so the lanes of a round update distinct arena nodes at each stencil
-// offset.  The layout is McNaughton's wrap-around rule: with the chunk's
-// particles listed cell after cell and S = max(largest cell, ceil(count/64)),
-// list entry q goes to lane q / S of round q % S.  A cell's entries are at
-// most S consecutive list positions, so they never meet in one round.
+// contiguously, and no two of them share a base cell, so the lanes of a round
+// update distinct arena nodes at each stencil offset.  The layout is
+// McNaughton's wrap-around rule: with the chunk's particles listed cell after
+// cell and S = max(largest cell, ceil(count/64)), list entry q goes to lane
+// q / S of round q % S.  A cell's entries are at most S consecutive list
+// positions, so they never meet in one round.  Hence n_r = nl = ceil(count/S)
+// for r < full = count - (nl - 1) S and nl - 1 after.
 struct ChunkDesc {
     int start;       // first slot (= first index of the chunk in sorted order)
     int count;       // particles in the chunk

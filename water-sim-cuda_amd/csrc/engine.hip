@@ -90,7 +90,6 @@ struct mpm_engine {
     int gather_src = 0;
 
     ChunkDesc* chunks = nullptr;
-    uint8_t* rounds = nullptr;  // particles per round of each chunk, at chunk.start + round
     int chunk_limit = 0;  // bound on the chunk count (sum over scenes of min(n_s, blocks_s) + n / CHUNK + 1)
     int chunk_alloc = 0;  // descriptors allocated
     int chunk_est = 0;    // chunk count at the last host synchronisation (sizes launches)
@@ -217,7 +216,6 @@ void free_particles(mpm_engine* h) {
         h->plane_cap[i] = 0;
     }
     dev_free(h->chunks);
-    dev_free(h->rounds);
     dev_free(h->d_pstart);
     dev_free(h->sc.keys[0]);
     dev_free(h->sc.keys[1]);
@@ -306,7 +304,7 @@ SceneTable scene_table(const mpm_engine* h) {
 }
 
 // Re-sorts the particles of (src, src_stride, src_orig) -- block buckets, the
-// chunk table, rounds -- into the alternate plane buffer, moving the planes
+// chunk table, the round layout -- into the alternate plane buffer, moving the planes
 // in `mask`.  Nothing here waits for the device: the chunk count stays in
 // device memory and is bounded by chunk_limit by construction.
 int sort_and_reorder(mpm_engine* h, const float* src, size_t src_stride, const int* src_orig, uint32_t mask) {
@@ -333,7 +331,6 @@ int sort_and_reorder(mpm_engine* h, const float* src, size_t src_stride, const i
     a.chunk_cap = h->chunk_cap;
     a.chunk_limit = h->chunk_limit;
     a.chunk_est = h->chunk_est;
-    a.rounds = h->rounds;
     launch_resort(a, h->stream);
     HIPCHK(hipGetLastError());
     h->pcur = dst;
@@ -367,7 +364,6 @@ TransferArgs make_args(mpm_engine* h) {
     a.stride = h->plane_cap[h->pcur];
     a.chunks = h->chunks;
     a.nchunks = h->sc.counters + 3;
-    a.rounds = h->rounds;
     const GridBuf& V = h->grid[h->vcur];
     const GridBuf& M = h->grid[1 - h->vcur];
     a.vel[0] = V.plane[1];
@@ -697,7 +693,7 @@ int init_particles_impl(mpm_engine* h, const int* counts, const float* soa26, bo
     HIPCHK(hipMemcpyAsync(tmp, soa26, sizeof(float) * MPM_PARTICLE_ATTRS * (size_t)n, hipMemcpyHostToDevice, h->stream));
     if (init_f) launch_init_F(tmp, n, h->stream);
     if ((rc = ensure_sort_scratch(h, (size_t)n)) || (rc = set_chunk_cap(h, h->chunk_cap)) ||
-        (rc = dev_alloc(&h->rounds, (size_t)n)) || (rc = ensure_planes(h, 0, (size_t)n)) ||
+        (rc = ensure_planes(h, 0, (size_t)n)) ||
         (rc = ensure_planes(h, 1, (size_t)n))) {
         dev_free(tmp);
         return rc;

@@ -30,7 +30,6 @@ struct TransferArgs {
     size_t stride;
     const ChunkDesc* chunks;
     const uint32_t* nchunks;  // chunk count, on the device
-    const uint8_t* rounds;    // particles per round, at chunk.start + round
     const float* vel[3];  // grid velocity planes read by G2P
     float* acc[4];        // grid mass/momentum planes written by P2G
     uint8_t* acc_flags;
@@ -137,7 +136,6 @@ struct ResortArgs {
     int chunk_cap;
     int chunk_limit;      // descriptors allocated (bounds the chunk count by construction)
     int chunk_est;        // chunk count estimate (sizes the reorder grid)
-    uint8_t* rounds;      // n entries
 };
 
 size_t radix_hist_elems(size_t n);

@@ -428,7 +428,6 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
                                                         const float* __restrict__ src, size_t src_stride,
                                                         const int* __restrict__ src_orig, float* __restrict__ dst,
                                                         size_t dst_stride, int* __restrict__ dst_orig,
-                                                        uint8_t* __restrict__ rounds,
                                                         uint32_t* __restrict__ rounds_total,
                                                         uint32_t* __restrict__ map, uint32_t mask) {
     static_assert(BLOCK_CELLS_N == 128 && RO_THREADS >= 128, "two waves scan the cell counts");
@@ -506,8 +505,6 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
             const uint32_t L = q / S, r = q - L * S;
             lst[(r < full ? r * nl : full * nl + (r - full) * (nl - 1u)) + L] = cell_rank[q];
         }
-        for (uint32_t r = (uint32_t)tid; r < S; r += RO_THREADS)
-            rounds[d.start + r] = (uint8_t)(r < full ? nl : nl - 1u);
         if (tid == 0) {
             ch[ci].S = (int)S;
             rsum += S;
@@ -610,11 +607,11 @@ void launch_resort(const ResortArgs& a, hipStream_t s) {
     if (a.mask)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_reorder<true>), dim3((unsigned)grid), dim3(RO_THREADS), lds, s, a.chunks,
                            a.counters, a.chunk_cap, a.keys[1], a.vals[1], a.src, a.src_stride, a.src_orig, a.dst,
-                           a.dst_stride, a.dst_orig, a.rounds, a.counters + 4, a.map, a.mask);
+                           a.dst_stride, a.dst_orig, a.counters + 4, a.map, a.mask);
     else
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_reorder<false>), dim3((unsigned)grid), dim3(RO_THREADS), lds, s, a.chunks,
                            a.counters, a.chunk_cap, a.keys[1], a.vals[1], a.src, a.src_stride, a.src_orig, a.dst,
-                           a.dst_stride, a.dst_orig, a.rounds, a.counters + 4, a.map, 0u);
+                           a.dst_stride, a.dst_orig, a.counters + 4, a.map, 0u);
 }
 
 }  // namespace mpm

@@ -320,10 +320,13 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
     __syncthreads();
 
     const int S = d.S;
-    // Round j is the n_j particles in slots [start + off_j, start + off_j + n_j).
-    // The next round's particle is loaded while the current one is
-    // processed, hiding HBM latency.
-    int nxt_n = (int)a.rounds[d.start];
+    // Round j is the n_j particles in slots [start + off_j, start + off_j + n_j):
+    // n_j = nl for j < full and nl - 1 after (the wrap-around layout).  The
+    // next round's particle is loaded while the current one is processed,
+    // hiding HBM latency.
+    const int nl = (d.count + S - 1) / S;
+    const int full = d.count - (nl - 1) * S;
+    int nxt_n = full > 0 ? nl : nl - 1;
     int nxt_off = 0;
     Particle nxt;
     bool nxt_valid = lane < nxt_n;
@@ -338,7 +341,7 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
         const uint32_t slot = nxt_slot;
         Particle p = nxt;
         nxt_off += nxt_n;
-        nxt_n = j + 1 < S ? (int)a.rounds[d.start + j + 1] : 0;
+        nxt_n = j + 1 < S ? (j + 1 < full ? nl : nl - 1) : 0;
         nxt_valid = lane < nxt_n;
         nxt_slot = (uint32_t)(d.start + nxt_off + lane);
         if (nxt_valid) load(nxt_slot, nxt);
