@@ -323,28 +323,38 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
     // Round j is the n_j particles in slots [start + off_j, start + off_j + n_j):
     // n_j = nl for j < full and nl - 1 after (the wrap-around layout).  The
     // next round's particle is loaded while the current one is processed,
-    // hiding HBM latency.
+    // hiding HBM latency; with GATHER the map entries are loaded one round
+    // earlier still, so the particle loads never wait for them.
     const int nl = (d.count + S - 1) / S;
     const int full = d.count - (nl - 1) * S;
-    int nxt_n = full > 0 ? nl : nl - 1;
+    auto round_n = [&](int j) { return j < S ? (j < full ? nl : nl - 1) : 0; };
+    int nxt_n = round_n(0);
     int nxt_off = 0;
     Particle nxt;
     bool nxt_valid = lane < nxt_n;
     uint32_t nxt_slot = (uint32_t)(d.start + lane);
-    auto load = [&](uint32_t slot, Particle& p) {
-        if constexpr (GATHER) load_particle(a.src, a.src_stride, a.map[slot], p, !DO_G2P);
-        else load_particle(a.part, a.stride, slot, p, !DO_G2P);
-    };
-    if (nxt_valid) load(nxt_slot, nxt);
+    uint32_t nn_src = 0u;  // GATHER: old slot of this lane's particle in the round after the next
+    if constexpr (GATHER) {
+        if (lane < round_n(1)) nn_src = a.map[d.start + nxt_n + lane];
+        if (nxt_valid) load_particle(a.src, a.src_stride, a.map[nxt_slot], nxt, !DO_G2P);
+    } else {
+        if (nxt_valid) load_particle(a.part, a.stride, nxt_slot, nxt, !DO_G2P);
+    }
     for (int j = 0; j < S; ++j) {
         const bool valid = nxt_valid;
         const uint32_t slot = nxt_slot;
         Particle p = nxt;
         nxt_off += nxt_n;
-        nxt_n = j + 1 < S ? (j + 1 < full ? nl : nl - 1) : 0;
+        nxt_n = round_n(j + 1);
         nxt_valid = lane < nxt_n;
         nxt_slot = (uint32_t)(d.start + nxt_off + lane);
-        if (nxt_valid) load(nxt_slot, nxt);
+        if constexpr (GATHER) {
+            const uint32_t si = nn_src;
+            if (lane < round_n(j + 2)) nn_src = a.map[d.start + nxt_off + nxt_n + lane];
+            if (nxt_valid) load_particle(a.src, a.src_stride, si, nxt, !DO_G2P);
+        } else {
+            if (nxt_valid) load_particle(a.part, a.stride, nxt_slot, nxt, !DO_G2P);
+        }
         if (valid) {
             if constexpr (DO_G2P) {
                 const int bx = base_of(p.x), by = base_of(p.y), bz = base_of(p.z);
