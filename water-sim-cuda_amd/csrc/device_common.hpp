@@ -73,8 +73,9 @@ enum : int {
 // contiguously, and no two of them share a base cell, so the lanes of a round
 // update distinct arena nodes at each stencil offset.  The layout is
 // McNaughton's wrap-around rule: with the chunk's particles listed cell after
-// cell and S = max(largest cell, ceil(count/64)), list entry q goes to lane
-// q / S of round q % S.  A cell's entries are at most S consecutive list
+// cell and S = max(largest cell, ceil(count/64)), list entry q goes to round
+// q % S (as if to lane q / S; the re-sort then orders each round's lanes by
+// LDS bank group).  A cell's entries are at most S consecutive list
 // positions, so they never meet in one round.  Hence n_r = nl = ceil(count/S)
 // for r < full = count - (nl - 1) S and nl - 1 after.
 struct ChunkDesc {

@@ -267,16 +267,19 @@ __device__ __forceinline__ uint64_t lanemask_le(int lane) { return (2ull << lane
 
 // Bucket key of a position: the rank of its base cell's 8 x 4 x 4 block (base
 // cells clamped into the grid) above the 7-bit index of the cell within the
-// block: the Morton index within its 4^3 half, then the half.  Both 4^3 flag
-// blocks of a block carry the block's rank.
+// block: the half (x bit 2), then the Morton index within the 4^3 half.  Both
+// 4^3 flag blocks of a block carry the block's rank.  With the half lowest,
+// the cells two apart in this order -- those a round's neighbouring lanes
+// hold when cells have about 8 particles -- form 2x2x2 groups, which sit on
+// distinct LDS bank groups in the transfer tile.
 __device__ __forceinline__ uint32_t bucket_key(const DevCfg& c, const uint32_t* __restrict__ brank, float x, float y,
                                                float z) {
     const int bx = min(max(base_of(x), 0), c.sx - 1);
     const int by = min(max(base_of(y), 0), c.sy - 1);
     const int bz = min(max(base_of(z), 0), c.sz - 1);
     const int fb = c.foff + (bx >> 2) + c.fbx * ((by >> 2) + c.fby * (bz >> 2));
-    const uint32_t cell = (uint32_t)((bx & 1) | ((by & 1) << 1) | ((bz & 1) << 2) | ((bx & 2) << 2) |
-                                     ((by & 2) << 3) | ((bz & 2) << 4) | ((bx & 4) << 4));
+    const uint32_t cell = (uint32_t)(((bx & 4) >> 2) | ((bx & 1) << 1) | ((by & 1) << 2) | ((bz & 1) << 3) |
+                                     ((bx & 2) << 3) | ((by & 2) << 4) | ((bz & 2) << 5));
     return (brank[fb] << BLOCK_BITS) | cell;
 }
 
@@ -405,8 +408,12 @@ constexpr int RO_THREADS = 256;
 // 8 x 4 x 4 block) and listed cell after cell, the members of a cell in
 // caller-index order, and the rounds laid out by the wrap-around rule
 // (ChunkDesc): with S = max(largest cell, ceil(count / 64)) rounds, list entry
-// q goes to lane q / S of round q % S; rounds 0 .. full-1 hold
-// nl = ceil(count / S) lanes and the others nl - 1.  Then map[new slot] = old
+// q goes to round q % S; rounds 0 .. full-1 hold nl = ceil(count / S)
+// particles and the others nl - 1.  Within a round the particles are ordered
+// by the 16-byte LDS bank group of their base slot in the transfer tile
+// (slot mod 8): all first members of the groups, then all second members, and
+// so on, so that runs of 8 lanes fall on distinct bank groups where the round
+// allows it.  Then map[new slot] = old
 // slot, and the caller indices and the planes in `mask` (bit a = attribute a)
 // move to the new slots (PLANES = false: mask is 0, only the map and the
 // caller indices; fewer registers, more chunks in flight).
@@ -494,16 +501,35 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
                 rank = 0u;
                 for (uint32_t k = 0; k < size; ++k) rank += org[lst[first + k] & 0xFFFFu] < g ? 1u : 0u;
             }
-            cell_rank[first + rank] = q;
+            cell_rank[first + rank] = q | (c << 16);
         }
         __syncthreads();
         const uint32_t n = (uint32_t)d.count;
         const uint32_t S = max(max(wmax[0], wmax[1]), (n + 63u) / 64u);
         const uint32_t nl = (n + S - 1u) / S;     // lanes of the first rounds (<= 64 as S >= n / 64)
         const uint32_t full = n - (nl - 1u) * S;  // rounds with nl lanes, 1 .. S
-        for (uint32_t q = (uint32_t)tid; q < n; q += RO_THREADS) {
-            const uint32_t L = q / S, r = q - L * S;
-            lst[(r < full ? r * nl : full * nl + (r - full) * (nl - 1u)) + L] = cell_rank[q];
+        // one wave per round: lane L holds list entry L * S + r
+        for (uint32_t r = (uint32_t)wave; r < S; r += RO_THREADS / 64) {
+            const uint32_t m = r < full ? nl : nl - 1u;
+            const bool act = (uint32_t)lane < m;
+            const uint32_t e = act ? cell_rank[(uint32_t)lane * S + r] : 0u;
+            const uint32_t c = e >> 16;
+            const uint32_t cx = ((c >> 1) & 1u) | (((c >> 4) & 1u) << 1) | ((c & 1u) << 2);
+            const uint32_t cy = ((c >> 2) & 1u) | (((c >> 5) & 1u) << 1);
+            const uint32_t cz = ((c >> 3) & 1u) | (((c >> 6) & 1u) << 1);
+            const uint32_t g = (cx + (uint32_t)TILE_PY * cy + (uint32_t)TILE_PZ * cz) & 7u;
+            uint32_t cg[8], rank = 0u;
+#pragma unroll
+            for (uint32_t h = 0; h < 8; ++h) {
+                const uint64_t mk = __ballot(act && g == h);
+                cg[h] = (uint32_t)__popcll(mk);
+                if (g == h) rank = (uint32_t)__popcll(mk & lanemask_lt(lane));
+            }
+            // members of lower rank in any group, then lower groups of the same rank
+            uint32_t pos = 0u;
+#pragma unroll
+            for (uint32_t h = 0; h < 8; ++h) pos += min(cg[h], rank) + ((h < g && cg[h] > rank) ? 1u : 0u);
+            if (act) lst[(r < full ? r * nl : full * nl + (r - full) * (nl - 1u)) + pos] = e & 0xFFFFu;
         }
         if (tid == 0) {
             ch[ci].S = (int)S;
