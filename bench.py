@@ -65,7 +65,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=80, help="untimed substeps")
     ap.add_argument("--scene", default="channel256")
     ap.add_argument("--resort-every", type=int, default=None, help="re-sort period (0: adaptive) [the scene's]")
-    ap.add_argument("--chunk", type=int, default=None)
+    ap.add_argument("--chunk", type=int, default=None, help="CHUNK [the scene's, else the engine default]")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events")
@@ -112,8 +112,8 @@ def main() -> int:
     # TIMING 2: HIP events around the transfer launches only (the roofline's
     # kernel); the per-stage table comes from a short TIMING 1 pass afterwards
     opts = {"RESORT_EVERY": resort, "TIMING": 0 if args.no_timing else 2, "GRAPH": 1 if args.graph else 0}
-    if args.chunk:
-        opts["CHUNK"] = args.chunk
+    if args.chunk or sc.get("chunk"):
+        opts["CHUNK"] = args.chunk or sc["chunk"]
     t0 = time.time()
     if batch:
         eng = mpm.Batch(sc["cfgs"], **opts)
@@ -161,6 +161,7 @@ def main() -> int:
     state = eng.getParticles()
     finite = bool(np.isfinite(state[0:6]).all())
     log(f"timed {args.steps} substeps in {elapsed:.3f}s; stages {st}; counters {ctr}; active nodes {active}")
+    chunk_used = eng.get_option("CHUNK")
     stage_tab = None
     if not args.no_timing:
         # every stage timed, after the timed region (events between all launches
@@ -260,6 +261,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": sc["name"], "grid": grid_desc, "particles": n,
                    "scenes": len(sc["cfgs"]) if batch else 1, "dt": cfg.DT, "resort_every": resort,
+                   "chunk": chunk_used,
                    "graph": bool(args.graph), "autotune": None if tuned is None else
                    {k: tuned[k] for k in ("CHUNK", "LAUNCH_PCT", "cached")},
                    "description": sc["description"], "state_finite": finite},
