@@ -114,7 +114,7 @@ int mpm_engine_destroy(mpm_engine* h);
  *                   engine.cu:57].  The sort changes only the order of the
  *                   float atomics, not the exact-arithmetic result.
  *   "CHUNK"         maximum particles per transfer work item, a multiple of
- *                   64 in [64, 2048] [1024]; changing it re-lays the particles
+ *                   64 in [64, 2048] [2048]; changing it re-lays the particles
  *   "LAUNCH_PCT"    transfer workgroups per 100 chunks, 1..100 [100]; below
  *                   100 each workgroup loops over several chunks
  *   "TIMING"        1 = record per-stage HIP events, 2 = around the transfer

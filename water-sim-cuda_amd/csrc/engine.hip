@@ -68,7 +68,7 @@ struct mpm_engine {
 
     int n = 0;
     bool have_particles = false;
-    int chunk_cap = 1024;
+    int chunk_cap = 2048;
     int launch_pct = 100;  // transfer workgroups per 100 chunks
     int resort_every = 1;
     int timing = 0;  // 0 off, 1 every stage, 2 the transfer stages only
