@@ -15,9 +15,10 @@
 // arena belongs to one wave; the 27 stencil nodes are visited in program
 // order and LDS executes one wave's operations in order, so a later offset
 // always sees an earlier offset's write.  Within one offset, lanes must hit
-// distinct nodes: lanes whose particles share a base node are found with a
-// ballot match and take turns.  The sums are the same float additions the
-// reference performs, in a different order.
+// distinct nodes: of the lanes whose particles share a base node (after
+// drifting since the last re-sort) one wins a claim byte and the others use
+// LDS float atomics.  The sums are the same float additions the reference
+// performs, in a different order.
 //
 // Three modes share the code:
 //   XFER_P2G   : p2GTransferScatter, kernels.cu:6-142
@@ -426,16 +427,16 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
                 }
             }
             // Lanes with distinct base nodes update the arena together: each
-            // pending lane writes its id into the claim byte of its base slot
-            // and reads it back; one writer per slot wins (LDS executes the
-            // wave's write before its read) and the others retry in another
-            // pass.  A round holds distinct base cells when the chunk is
-            // laid out, so extra passes only come from particles whose base
-            // cell changed since the last re-sort.  The first pass stands
-            // outside the retry loop, and the retry passes launder their
-            // inputs, so that the compiler keeps the common path's
-            // contributions inside the update chain instead of hoisting them
-            // (and their registers) above a loop.
+            // lane writes its id into the claim byte of its base slot and
+            // reads it back; one writer per slot wins (LDS executes the
+            // wave's write before its read) and runs the read-modify-write
+            // chain.  A round holds distinct base cells when the chunk is
+            // laid out, so only particles whose base cell changed since the
+            // last re-sort lose; they add their contributions with LDS float
+            // atomics afterwards (slow per lane, but one or two lanes need no
+            // 27-step dependent chain).  Their inputs are laundered so that
+            // the compiler does not share the two paths' contributions (it
+            // would compute them all above the claim test).
             auto arena_chain = [&](const Particle& pp, const P2GPrep& qq) {
                 float4* base = ar + li;
                 p2g_scatter(c, pp, qq, [&](int i, int jj, int k, const P2GNode& nd) {
@@ -451,26 +452,27 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
             const bool mine = in_tile && claim[li] == (uint8_t)lane;
             asm volatile("" ::: "memory");
             if (mine) arena_chain(p, pq);
-            bool pending = in_tile && !mine;
-            while (__ballot(pending)) {
-                if (pending) claim[li] = (uint8_t)lane;
-                asm volatile("" ::: "memory");
-                const bool won = pending && claim[li] == (uint8_t)lane;
-                asm volatile("" ::: "memory");
-                if (won) {
-                    atomicAdd(a.overflow + 1, 1u);
-                    Particle p2 = p;
-                    P2GPrep q2 = pq;
-                    asm volatile("" : "+v"(p2.vx), "+v"(p2.vy), "+v"(p2.vz));
+            const bool lost = in_tile && !mine;
+            if (lost) {
+                atomicAdd(a.overflow + 1, 1u);
+                Particle p2 = p;
+                P2GPrep q2 = pq;
+                asm volatile("" : "+v"(p2.vx), "+v"(p2.vy), "+v"(p2.vz));
 #pragma unroll
-                    for (int t = 0; t < 9; ++t) asm volatile("" : "+v"(q2.C[t]));
+                for (int t = 0; t < 9; ++t) asm volatile("" : "+v"(q2.C[t]));
 #pragma unroll
-                    for (int t = 0; t < 3; ++t)
-                        asm volatile("" : "+v"(q2.wx[t]), "+v"(q2.wy[t]), "+v"(q2.wz[t]), "+v"(q2.dx[t]),
-                                     "+v"(q2.dy[t]), "+v"(q2.dz[t]));
-                    arena_chain(p2, q2);
-                }
-                pending = pending && !won;
+                for (int t = 0; t < 3; ++t)
+                    asm volatile("" : "+v"(q2.wx[t]), "+v"(q2.wy[t]), "+v"(q2.wz[t]), "+v"(q2.dx[t]),
+                                 "+v"(q2.dy[t]), "+v"(q2.dz[t]));
+                float4* base = ar + li;
+                p2g_scatter(c, p2, q2, [&](int i, int jj, int k, const P2GNode& nd) {
+                    float* n = reinterpret_cast<float*>(base + (i + TILE_PY * jj + TILE_PZ * k));
+                    const float4 v = nd.value();
+                    atomicAdd(n + 0, v.x);
+                    atomicAdd(n + 1, v.y);
+                    atomicAdd(n + 2, v.z);
+                    atomicAdd(n + 3, v.w);
+                });
             }
         }
     }
