@@ -268,6 +268,10 @@ __device__ __forceinline__ void load_particle(const float* __restrict__ part, si
     }
 }
 
+__device__ __forceinline__ bool node_in_tile(int tx, int ty, int tz) {
+    return (unsigned)tx < (unsigned)TILE_DX && (unsigned)ty < (unsigned)TILE_DY && (unsigned)tz < (unsigned)TILE_DZ;
+}
+
 __device__ __forceinline__ bool stencil_in_tile(int lx, int ly, int lz) {
     return (unsigned)lx <= (unsigned)(TILE_DX - 3) && (unsigned)ly <= (unsigned)(TILE_DY - 3) &&
            (unsigned)lz <= (unsigned)(TILE_DZ - 3);
@@ -375,11 +379,18 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
                         },
                         wx, wy, wz, dx, dy, dz, vel, C);
                 } else {
+                    // stencil partly outside the tile: nodes inside it from the
+                    // tile (the same values), the others from global memory
                     g2p_gather(
                         [&](int i, int jj, int k) {
-                            int x = bx + i, y = by + jj, z = bz + k;
+                            const int x = bx + i, y = by + jj, z = bz + k;
+                            const int tx = x - d.ox, ty = y - d.oy, tz = z - d.oz;
+                            if (node_in_tile(tx, ty, tz)) {
+                                const float4 v = vt[tx + TILE_PY * ty + TILE_PZ * tz];
+                                return make_float3(v.x, v.y, v.z);
+                            }
                             if (!inb(c, x, y, z)) return make_float3(0.f, 0.f, 0.f);
-                            int g = gidx(c, x, y, z);
+                            const int g = gidx(c, x, y, z);
                             return make_float3(a.vel[0][g], a.vel[1][g], a.vel[2][g]);
                         },
                         wx, wy, wz, dx, dy, dz, vel, C);
@@ -412,16 +423,29 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
                 li = in_tile ? (uint32_t)(lx + TILE_PY * ly + TILE_PZ * lz) : 0u;
                 p2g_prepare(c, p, bx, by, bz, pq);
                 if (!in_tile) {
+                    // stencil partly outside the tile: LDS float atomics for
+                    // the nodes inside it (before any lane's read-modify-write
+                    // chain below, in program order), global atomics for the
+                    // others
                     atomicAdd(a.overflow, 1u);
                     p2g_scatter(c, p, pq, [&](int i, int jj, int k, const P2GNode& nd) {
-                        int x = bx + i, y = by + jj, z = bz + k;
+                        const int x = bx + i, y = by + jj, z = bz + k;
+                        const int tx = x - d.ox, ty = y - d.oy, tz = z - d.oz;
+                        const float4 v = nd.value();
+                        if (node_in_tile(tx, ty, tz)) {
+                            float* n = reinterpret_cast<float*>(ar + (tx + TILE_PY * ty + TILE_PZ * tz));
+                            atomicAdd(n + 0, v.x);
+                            atomicAdd(n + 1, v.y);
+                            atomicAdd(n + 2, v.z);
+                            atomicAdd(n + 3, v.w);
+                            return;
+                        }
                         if (!inb(c, x, y, z)) return;
-                        int g = gidx(c, x, y, z);
-                        const float4 d = nd.value();
-                        atomic_add_global(a.acc[0] + g, d.x);
-                        atomic_add_global(a.acc[1] + g, d.y);
-                        atomic_add_global(a.acc[2] + g, d.z);
-                        atomic_add_global(a.acc[3] + g, d.w);
+                        const int g = gidx(c, x, y, z);
+                        atomic_add_global(a.acc[0] + g, v.x);
+                        atomic_add_global(a.acc[1] + g, v.y);
+                        atomic_add_global(a.acc[2] + g, v.z);
+                        atomic_add_global(a.acc[3] + g, v.w);
                         a.acc_flags[flag_block(c, x, y, z)] = 1;
                     });
                 }
