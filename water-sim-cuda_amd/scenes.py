@@ -74,7 +74,7 @@ def channel256(physical: bool = True, seed: int = 3) -> dict:
     b = m.scene_fill("box", (150.0, 120.0, 150.0), (214.0, 184.0, 214.0), seed + 1000)
     parts = np.concatenate([a, b], axis=1)
     tris = m.mesh_u_channel((6.3, 20.3, 6.3), (249.7, 90.3, 249.7), 4.0, 6.25)
-    return dict(name="channel256", cfg=cfg, particles=parts, sdf=None, triangles=tris, resort_every=6,
+    return dict(name="channel256", cfg=cfg, particles=parts, sdf=None, triangles=tris, resort_every=5,
                 frames=100, description="256^3, boxes 96x128x96 + 64^3 cells into a U-channel mesh")
 
 
