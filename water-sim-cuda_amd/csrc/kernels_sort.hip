@@ -424,8 +424,9 @@ constexpr int RO_THREADS = 256;
 // caused by particles that moved, so the map is close to the identity and the
 // next transfer's reads through it stay coalesced.  Cells with more than
 // RO_SORT_MAX members (far denser than a fluid) keep the atomics' order.
-// Dynamic LDS: 4 x cap words (cap <= 2048 as chunk positions take 16 bits
-// in the list entries and 64 KB bound the workgroup's LDS).
+// Dynamic LDS: 3 x cap words (cap <= 2048: chunk positions take 16 bits in
+// the list entries); the old slots are read from `vals` again at the end
+// rather than held, which leaves room for more chunks per CU.
 constexpr int RO_SORT_MAX = 64;
 template <bool PLANES>
 __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ ch,
@@ -441,9 +442,8 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
     extern __shared__ uint32_t ro_lds[];
     uint32_t* cell_rank = ro_lds;          // [cap] cell | atomic rank << BLOCK_BITS per chunk position;
                                            //       then the list in final order (chunk positions)
-    uint32_t* slot_of = ro_lds + cap;      // [cap] old slot per chunk position
-    uint32_t* org = ro_lds + 2 * cap;      // [cap] caller index per chunk position
-    uint32_t* lst = ro_lds + 3 * cap;      // [cap] list in atomic order (position | cell << 16);
+    uint32_t* org = ro_lds + cap;          // [cap] caller index per chunk position
+    uint32_t* lst = ro_lds + 2 * cap;      // [cap] list in atomic order (position | cell << 16);
                                            //       then the chunk position of each new slot
     __shared__ uint32_t cnt[BLOCK_CELLS_N];
     __shared__ uint32_t cfirst[BLOCK_CELLS_N];
@@ -459,7 +459,6 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
         for (int q = tid; q < d.count; q += RO_THREADS) {
             const uint32_t si = vals[d.start + q];
             cell_rank[q] = keys[d.start + q] & (uint32_t)(BLOCK_CELLS_N - 1);
-            slot_of[q] = si;
             org[q] = src_orig ? (uint32_t)src_orig[si] : si;
         }
         __syncthreads();
@@ -538,7 +537,7 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
         __syncthreads();
         for (int o = tid; o < d.count; o += RO_THREADS) {
             const uint32_t q = lst[o];
-            const uint32_t si = slot_of[q];
+            const uint32_t si = vals[d.start + q];  // the old slot, read again (L2-resident)
             const size_t t = (size_t)d.start + o;
             map[t] = si;
             if constexpr (PLANES) {
@@ -629,7 +628,7 @@ void launch_resort(const ResortArgs& a, hipStream_t s) {
                        a.chunk_limit);
     if (n <= 0) return;
     const int grid = std::max(1, std::min(a.chunk_est, 16384));
-    const size_t lds = (size_t)4 * a.chunk_cap * sizeof(uint32_t);
+    const size_t lds = (size_t)3 * a.chunk_cap * sizeof(uint32_t);
     if (a.mask)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_reorder<true>), dim3((unsigned)grid), dim3(RO_THREADS), lds, s, a.chunks,
                            a.counters, a.chunk_cap, a.keys[1], a.vals[1], a.src, a.src_stride, a.src_orig, a.dst,
