@@ -128,14 +128,6 @@ def main() -> int:
         else:
             eng.initSDF(sc["triangles"])
     log(f"engine ready ({time.time() - t0:.2f}s)")
-    tuned = None
-    if args.autotune:
-        from importlib import import_module
-
-        tuned = import_module("water-sim-cuda_amd.autotune").autotune(eng, cache=args.tune_cache)
-        log(f"autotune: CHUNK={tuned['CHUNK']} LAUNCH_PCT={tuned['LAUNCH_PCT']} cached={tuned['cached']} "
-            f"trials {tuned.get('trials')}")
-
     # The reference's frame loop: SUBSTEPS substeps per call (Simulation.cpp:42-54).
     frame = args.frame or int(cfg.SUBSTEPS) or 20
     nframes = max(1, -(-args.steps // frame))
@@ -143,6 +135,18 @@ def main() -> int:
     for _ in range(wframes):  # with --graph the first call captures the frame graph
         eng.steps(frame)
     eng.synchronize()
+    tuned = None
+    if args.autotune:
+        # trials on the warmed-up state (restored after each trial), then one
+        # more warm-up frame in the chosen setting
+        from importlib import import_module
+
+        tuned = import_module("water-sim-cuda_amd.autotune").autotune(eng, cache=args.tune_cache)
+        log(f"autotune: CHUNK={tuned['CHUNK']} LAUNCH_PCT={tuned['LAUNCH_PCT']} cached={tuned['cached']} "
+            f"trials {tuned.get('trials')}")
+        eng.steps(frame)
+        wframes += 1
+        eng.synchronize()
     eng.stage_times()  # drop warm-up events
     log(f"warmup {wframes} x {frame} substeps done")
     args.steps, args.warmup = nframes * frame, wframes * frame
