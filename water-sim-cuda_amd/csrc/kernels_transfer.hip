@@ -268,6 +268,16 @@ __device__ __forceinline__ void load_particle(const float* __restrict__ part, si
     }
 }
 
+// A node contribution added to an arena slot with LDS float atomics
+// (ds_add_f32), for the few lanes that cannot use the read-modify-write chain.
+__device__ __forceinline__ void lds_add4(float4* slot, const float4 v) {
+    float* n = reinterpret_cast<float*>(slot);
+    atomicAdd(n + 0, v.x);
+    atomicAdd(n + 1, v.y);
+    atomicAdd(n + 2, v.z);
+    atomicAdd(n + 3, v.w);
+}
+
 __device__ __forceinline__ bool node_in_tile(int tx, int ty, int tz) {
     return (unsigned)tx < (unsigned)TILE_DX && (unsigned)ty < (unsigned)TILE_DY && (unsigned)tz < (unsigned)TILE_DZ;
 }
@@ -433,11 +443,7 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
                         const int tx = x - d.ox, ty = y - d.oy, tz = z - d.oz;
                         const float4 v = nd.value();
                         if (node_in_tile(tx, ty, tz)) {
-                            float* n = reinterpret_cast<float*>(ar + (tx + TILE_PY * ty + TILE_PZ * tz));
-                            atomicAdd(n + 0, v.x);
-                            atomicAdd(n + 1, v.y);
-                            atomicAdd(n + 2, v.z);
-                            atomicAdd(n + 3, v.w);
+                            lds_add4(ar + (tx + TILE_PY * ty + TILE_PZ * tz), v);
                             return;
                         }
                         if (!inb(c, x, y, z)) return;
@@ -490,12 +496,7 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
                                  "+v"(q2.dy[t]), "+v"(q2.dz[t]));
                 float4* base = ar + li;
                 p2g_scatter(c, p2, q2, [&](int i, int jj, int k, const P2GNode& nd) {
-                    float* n = reinterpret_cast<float*>(base + (i + TILE_PY * jj + TILE_PZ * k));
-                    const float4 v = nd.value();
-                    atomicAdd(n + 0, v.x);
-                    atomicAdd(n + 1, v.y);
-                    atomicAdd(n + 2, v.z);
-                    atomicAdd(n + 3, v.w);
+                    lds_add4(base + (i + TILE_PY * jj + TILE_PZ * k), nd.value());
                 });
             }
         }
