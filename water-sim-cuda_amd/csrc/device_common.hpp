@@ -78,6 +78,7 @@ enum : int {
 // LDS bank group).  A cell's entries are at most S consecutive list
 // positions, so they never meet in one round.  Hence n_r = nl = ceil(count/S)
 // for r < full = count - (nl - 1) S and nl - 1 after.
+constexpr int CHUNK_CAP_MAX = 2048;  // largest CHUNK (the re-sort's LDS and per-thread items)
 struct ChunkDesc {
     int start;       // first slot (= first index of the chunk in sorted order)
     int count;       // particles in the chunk

@@ -836,7 +836,7 @@ int mpm_engine_set_option(mpm_engine* h, const char* key, int64_t value) {
         if (value < 1) return fail(MPM_EINVAL, "RESORT_EVERY must be >= 1");
         h->resort_every = (int)std::min<int64_t>(value, 1 << 30);
     } else if (k == "CHUNK") {
-        if (value < 64 || value > 2048 || (value % 64) != 0)
+        if (value < 64 || value > CHUNK_CAP_MAX || (value % 64) != 0)
             return fail(MPM_EINVAL, "CHUNK must be a multiple of 64 in [64, 2048]");
         if (value == h->chunk_cap) return MPM_OK;
         if (!h->have_particles || h->n == 0) {

@@ -12,8 +12,8 @@
 //   2. a scan over the blocks of (particles, chunks);
 //   3. a scatter into block order;
 //   4. the chunk table, one thread per block;
-//   5. one workgroup per chunk: a counting sort by cell in LDS, the round
-//      layout, and the gather of the planes.
+//   5. persistent workgroups over the chunks: a counting sort by cell in LDS,
+//      the round layout, and the gather of the planes.
 // Particles move little between re-sorts, so consecutive slots mostly share a
 // block and the buckets see about one atomic per wave.  The order of the
 // particles of a block is not fixed (atomics), which changes only the order
@@ -402,8 +402,10 @@ __global__ __launch_bounds__(256) void k_bucket_chunks(uint32_t* __restrict__ bc
 }
 
 constexpr int RO_THREADS = 256;
+constexpr int RO_ITEMS = CHUNK_CAP_MAX / RO_THREADS;  // chunk positions per thread
+static_assert(RO_ITEMS * RO_THREADS == CHUNK_CAP_MAX, "every chunk position has a thread");
 
-// 5. One workgroup per chunk (grid-stride over the chunk table).  In LDS: the
+// 5. Persistent workgroups, each taking every gridDim-th chunk.  In LDS: the
 // chunk's particles counted and ranked by cell (at most 128 cells: one
 // 8 x 4 x 4 block) and listed cell after cell, the members of a cell in
 // caller-index order, and the rounds laid out by the wrap-around rule
@@ -413,10 +415,15 @@ constexpr int RO_THREADS = 256;
 // by the 16-byte LDS bank group of their base slot in the transfer tile
 // (slot mod 8): all first members of the groups, then all second members, and
 // so on, so that runs of 8 lanes fall on distinct bank groups where the round
-// allows it.  Then map[new slot] = old
-// slot, and the caller indices and the planes in `mask` (bit a = attribute a)
-// move to the new slots (PLANES = false: mask is 0, only the map and the
-// caller indices; fewer registers, more chunks in flight).
+// allows it.  Then map[new slot] = old slot, and the caller indices and the
+// planes in `mask` (bit a = attribute a) move to the new slots (PLANES =
+// false: mask is 0, only the map and the caller indices).
+//
+// Chunk position q belongs to thread q % RO_THREADS, which keeps its old slot
+// in a register from the load to the final stores.  The old slots, keys and
+// caller indices of a workgroup's next chunk are loaded while it lays out the
+// current one (the barriers wait for LDS operations only), so the memory
+// latency of the loads overlaps the layout work.
 //
 // Ordering a cell's members by caller index (not by the order of the LDS
 // atomics) makes the layout a function of the particles' cells alone: a
@@ -425,8 +432,7 @@ constexpr int RO_THREADS = 256;
 // next transfer's reads through it stay coalesced.  Cells with more than
 // RO_SORT_MAX members (far denser than a fluid) keep the atomics' order.
 // Dynamic LDS: 3 x cap words (cap <= 2048: chunk positions take 16 bits in
-// the list entries); the old slots are read from `vals` again at the end
-// rather than held, which leaves room for more chunks per CU.
+// the list entries).
 constexpr int RO_SORT_MAX = 64;
 template <bool PLANES>
 __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ ch,
@@ -438,73 +444,111 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
                                                         size_t dst_stride, int* __restrict__ dst_orig,
                                                         uint32_t* __restrict__ rounds_total,
                                                         uint32_t* __restrict__ map, uint32_t mask) {
-    static_assert(BLOCK_CELLS_N == 128 && RO_THREADS >= 128, "two waves scan the cell counts");
+    static_assert(BLOCK_CELLS_N == 128 && RO_THREADS >= 128, "one wave scans the cell counts, two per lane");
     extern __shared__ uint32_t ro_lds[];
     uint32_t* cell_rank = ro_lds;          // [cap] cell | atomic rank << BLOCK_BITS per chunk position;
-                                           //       then the list in final order (chunk positions)
+                                           //       then the list in final order (position | cell << 16)
     uint32_t* org = ro_lds + cap;          // [cap] caller index per chunk position
     uint32_t* lst = ro_lds + 2 * cap;      // [cap] list in atomic order (position | cell << 16);
-                                           //       then the chunk position of each new slot
-    __shared__ uint32_t cnt[BLOCK_CELLS_N];
+                                           //       then the new slot (from start) of each chunk position
+    __shared__ uint32_t cnt[BLOCK_CELLS_N];  // zero between chunks
     __shared__ uint32_t cfirst[BLOCK_CELLS_N];
-    __shared__ uint32_t wtot[2], wmax[2];
+    __shared__ uint32_t cmax;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t nch = counters[3];
     uint32_t rsum = 0u;
+    // the next chunk's old slots, keys and caller indices, position tid + RO_THREADS i
+    uint32_t nsi[RO_ITEMS] = {}, nkey[RO_ITEMS] = {}, norg[RO_ITEMS] = {};
+    uint32_t si[RO_ITEMS];  // the current chunk's old slots
+    int nstart = 0, ncount = 0;
+    // (start, count) of chunk c, (0, 0) past the last chunk; read two chunks
+    // ahead so that no chunk waits for its descriptor
+    auto desc_of = [&](uint32_t c) { return c < nch ? make_int2(ch[c].start, ch[c].count) : make_int2(0, 0); };
+    auto load_slots = [&](int2 d) {
+        nstart = __builtin_amdgcn_readfirstlane(d.x);
+        ncount = __builtin_amdgcn_readfirstlane(d.y);
+        // Unconditional loads (positions past the chunk read entry 0, which
+        // exists as n > 0): a load under a branch would be copied into the
+        // loop-carried register, and the copy waits for the load.
+#pragma unroll
+        for (int i = 0; i < RO_ITEMS; ++i) {
+            const int q = tid + RO_THREADS * i;
+            const int at = q < ncount ? nstart + q : 0;
+            nsi[i] = vals[at];
+            nkey[i] = keys[at];
+        }
+    };
+    auto load_orig = [&]() {  // old slots are < n
+        if (src_orig) {
+#pragma unroll
+            for (int i = 0; i < RO_ITEMS; ++i) norg[i] = (uint32_t)src_orig[nsi[i]];
+        } else {
+#pragma unroll
+            for (int i = 0; i < RO_ITEMS; ++i) norg[i] = nsi[i];
+        }
+    };
+    if (tid < BLOCK_CELLS_N) cnt[tid] = 0u;
+    load_slots(desc_of(blockIdx.x));
+    int2 dnext = desc_of(blockIdx.x + gridDim.x);
+    load_orig();
+    __syncthreads();
     for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
-        const ChunkDesc d = uniform(ch[ci]);
-        if (tid < BLOCK_CELLS_N) cnt[tid] = 0u;
-        __syncthreads();
-#pragma unroll 4
-        for (int q = tid; q < d.count; q += RO_THREADS) {
-            const uint32_t si = vals[d.start + q];
-            cell_rank[q] = keys[d.start + q] & (uint32_t)(BLOCK_CELLS_N - 1);
-            org[q] = src_orig ? (uint32_t)src_orig[si] : si;
+        const int start = nstart, count = ncount;
+        // Wait for the loaded values on every path (an empty asm reading
+        // them): otherwise the compiler treats their loads as possibly still
+        // in flight after the guarded uses below and waits for the next
+        // chunk's loads when it reuses these registers.
+#pragma unroll
+        for (int i = 0; i < RO_ITEMS; ++i) asm volatile("" ::"v"(nsi[i]), "v"(nkey[i]), "v"(norg[i]));
+#pragma unroll
+        for (int i = 0; i < RO_ITEMS; ++i) {
+            const int q = tid + RO_THREADS * i;
+            si[i] = nsi[i];
+            if (q < count) {
+                const uint32_t c = nkey[i] & (uint32_t)(BLOCK_CELLS_N - 1);
+                org[q] = norg[i];
+                cell_rank[q] = c | (atomicAdd(&cnt[c], 1u) << BLOCK_BITS);
+            }
         }
+        load_slots(dnext);  // the next chunk's, in flight during this chunk's layout
+        dnext = desc_of(ci + 2u * gridDim.x);
         __syncthreads();
-        for (int q = tid; q < d.count; q += RO_THREADS) {
-            const uint32_t c = cell_rank[q];
-            cell_rank[q] = c | (atomicAdd(&cnt[c], 1u) << BLOCK_BITS);
-        }
-        __syncthreads();
-        if (tid < BLOCK_CELLS_N) {  // waves 0 and 1: 64 cells each
-            const uint32_t size = cnt[tid];
-            uint32_t x = size, m = size;
+        if (tid < 64) {  // wave 0: cells 2 lane and 2 lane + 1
+            const uint32_t a = cnt[2 * lane], b = cnt[2 * lane + 1];
+            uint32_t x = a + b, m = max(a, b);
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
                 const uint32_t y = __shfl_up(x, o);
                 if (lane >= o) x += y;
                 m = max(m, (uint32_t)__shfl_xor((int)m, o));
             }
-            cfirst[tid] = x - size;  // within the wave's 64 cells
-            if (lane == 63) {
-                wtot[wave] = x;
-                wmax[wave] = m;
-            }
+            cfirst[2 * lane] = x - a - b;
+            cfirst[2 * lane + 1] = x - b;
+            if (lane == 0) cmax = m;
         }
         __syncthreads();
-        if (tid >= 64 && tid < BLOCK_CELLS_N) cfirst[tid] += wtot[0];  // wave 1's cells follow wave 0's
-        __syncthreads();
-        for (int q = tid; q < d.count; q += RO_THREADS) {
+        for (int q = tid; q < count; q += RO_THREADS) {
             const uint32_t v = cell_rank[q], c = v & (uint32_t)(BLOCK_CELLS_N - 1);
             lst[cfirst[c] + (v >> BLOCK_BITS)] = (uint32_t)q | (c << 16);
         }
         __syncthreads();
         // each list entry's rank among its cell's members by caller index
-        for (int p = tid; p < d.count; p += RO_THREADS) {
+        for (int p = tid; p < count; p += RO_THREADS) {
             const uint32_t e = lst[p], q = e & 0xFFFFu, c = e >> 16;
             const uint32_t first = cfirst[c], size = cnt[c];
             uint32_t rank = (uint32_t)p - first;
             if (size <= (uint32_t)RO_SORT_MAX) {
                 const uint32_t g = org[q];
                 rank = 0u;
+#pragma unroll 4
                 for (uint32_t k = 0; k < size; ++k) rank += org[lst[first + k] & 0xFFFFu] < g ? 1u : 0u;
             }
             cell_rank[first + rank] = q | (c << 16);
         }
+        load_orig();  // the next chunk's slots have arrived by now
         __syncthreads();
-        const uint32_t n = (uint32_t)d.count;
-        const uint32_t S = max(max(wmax[0], wmax[1]), (n + 63u) / 64u);
+        const uint32_t n = (uint32_t)count;
+        const uint32_t S = max(cmax, (n + 63u) / 64u);
         const uint32_t nl = (n + S - 1u) / S;     // lanes of the first rounds (<= 64 as S >= n / 64)
         const uint32_t full = n - (nl - 1u) * S;  // rounds with nl lanes, 1 .. S
         // one wave per round: lane L holds list entry L * S + r
@@ -528,29 +572,28 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
             uint32_t pos = 0u;
 #pragma unroll
             for (uint32_t h = 0; h < 8; ++h) pos += min(cg[h], rank) + ((h < g && cg[h] > rank) ? 1u : 0u);
-            if (act) lst[(r < full ? r * nl : full * nl + (r - full) * (nl - 1u)) + pos] = e & 0xFFFFu;
+            if (act) lst[e & 0xFFFFu] = (r < full ? r * nl : full * nl + (r - full) * (nl - 1u)) + pos;
         }
         if (tid == 0) {
             ch[ci].S = (int)S;
             rsum += S;
         }
         __syncthreads();
-        for (int o = tid; o < d.count; o += RO_THREADS) {
-            const uint32_t q = lst[o];
-            const uint32_t si = vals[d.start + q];  // the old slot, read again (L2-resident)
-            const size_t t = (size_t)d.start + o;
-            map[t] = si;
-            if constexpr (PLANES) {
-                float v[A_STATE];
 #pragma unroll
-                for (int a = 0; a < A_STATE; ++a)
-                    if (mask & (1u << a)) v[a] = src[(size_t)a * src_stride + si];
+        for (int i = 0; i < RO_ITEMS; ++i) {
+            const int q = tid + RO_THREADS * i;
+            if (q < count) {
+                const size_t t = (size_t)start + lst[q];
+                map[t] = si[i];
+                if constexpr (PLANES) {
 #pragma unroll
-                for (int a = 0; a < A_STATE; ++a)
-                    if (mask & (1u << a)) dst[(size_t)a * dst_stride + t] = v[a];
+                    for (int a = 0; a < A_STATE; ++a)
+                        if (mask & (1u << a)) dst[(size_t)a * dst_stride + t] = src[(size_t)a * src_stride + si[i]];
+                }
+                dst_orig[t] = (int)org[q];
             }
-            dst_orig[t] = (int)org[q];
         }
+        if (tid < BLOCK_CELLS_N) cnt[tid] = 0u;
         __syncthreads();  // the LDS arrays are reused by the next chunk
     }
     if (tid == 0 && rsum) atomicAdd(rounds_total, rsum);
@@ -592,9 +635,13 @@ int radix_sort_pairs(SortScratch& sc, int n, int key_bits, hipStream_t s) {
         if (blocks > 1024) blocks = 1024;
         hipLaunchKernelGGL(k_key_or_and, dim3(blocks), dim3(256), 0, s, sc.keys[0], n, sc.counters);
         uint32_t oa[2] = {0u, 0u};
-        hipMemcpyAsync(oa, sc.counters, sizeof(oa), hipMemcpyDeviceToHost, s);
-        hipStreamSynchronize(s);
-        diff = oa[0] ^ oa[1];
+        // On a failed read-back every digit is sorted (always correct); the
+        // error stays pending for the caller's next status check.
+        if (hipMemcpyAsync(oa, sc.counters, sizeof(oa), hipMemcpyDeviceToHost, s) == hipSuccess &&
+            hipStreamSynchronize(s) == hipSuccess)
+            diff = oa[0] ^ oa[1];
+        else
+            diff = 0xFFFFFFFFu;
     }
     const int ntiles = (n + RS_TILE - 1) / RS_TILE;
     int r = 0;
@@ -627,8 +674,17 @@ void launch_resort(const ResortArgs& a, hipStream_t s) {
                        bt.block, a.block_scene, a.scenes.cfgs, bt.nb, a.chunk_cap, a.counters, a.chunks,
                        a.chunk_limit);
     if (n <= 0) return;
-    const int grid = std::max(1, std::min(a.chunk_est, 16384));
+    // persistent workgroups: as many as fit on the device at once
     const size_t lds = (size_t)3 * a.chunk_cap * sizeof(uint32_t);
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, a.mask ? (const void*)k_reorder<true>
+                                                                     : (const void*)k_reorder<false>,
+                                                     RO_THREADS, lds) != hipSuccess)
+        cus = per_cu = 0;
+    const int resident = std::max(1, cus * per_cu);
+    const int grid = std::max(1, std::min(a.chunk_est, resident));
     if (a.mask)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_reorder<true>), dim3((unsigned)grid), dim3(RO_THREADS), lds, s, a.chunks,
                            a.counters, a.chunk_cap, a.keys[1], a.vals[1], a.src, a.src_stride, a.src_orig, a.dst,
