@@ -73,6 +73,8 @@ def main() -> int:
     ap.add_argument("--frame", type=int, default=0, help="substeps per steps() call [the scene's SUBSTEPS]")
     ap.add_argument("--autotune", action="store_true", help="pick CHUNK and LAUNCH_PCT by timing trials first")
     ap.add_argument("--tune-cache", default=None, help="JSON cache file for --autotune")
+    ap.add_argument("--tune-online", type=int, default=0, metavar="FRAMES",
+                    help="keep tuning CHUNK in the timed run: one trial frame every FRAMES frames (0: off)")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "g2p2g_channel256_traffic.txt"),
                     help="rocprofv3-derived HBM bytes per launch of the fused kernel (key: value text)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-corrector evaluation")
@@ -151,12 +153,22 @@ def main() -> int:
     log(f"warmup {wframes} x {frame} substeps done")
     args.steps, args.warmup = nframes * frame, wframes * frame
 
+    tuner = None
+    if args.tune_online > 0:
+        from importlib import import_module
+
+        tuner = import_module("water-sim-cuda_amd.autotune").OnlineTuner(eng, frame, period=args.tune_online)
     eng.synchronize()
     t0 = time.perf_counter()
     for _ in range(nframes):
-        eng.steps(frame)
+        if tuner is not None:
+            tuner.frame()  # trial frames and CHUNK changes are part of the timed run
+        else:
+            eng.steps(frame)
     eng.synchronize()
     t1 = time.perf_counter()
+    if tuner is not None:
+        log(f"online tuning: {tuner.trials} trials, {tuner.switches} switches, history {tuner.history}")
     elapsed = t1 - t0
     st = eng.stage_times()
     ctr = eng.counters()
@@ -268,6 +280,8 @@ def main() -> int:
                    "chunk": chunk_used,
                    "graph": bool(args.graph), "autotune": None if tuned is None else
                    {k: tuned[k] for k in ("CHUNK", "LAUNCH_PCT", "cached")},
+                   "online_tune": None if tuner is None else
+                   {"period_frames": tuner.period, "trials": tuner.trials, "switches": tuner.switches},
                    "description": sc["description"], "state_finite": finite},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -194,3 +194,30 @@ def test_autotune_keeps_state_and_caches(gpu, small_batch, tmp_path):
     res2 = at.autotune(e, cache=cache)
     assert res2["cached"] and res2["CHUNK"] == res["CHUNK"]
     e.close()
+
+
+def test_online_tuner_runs_the_frames(gpu, small_batch):
+    """Trial frames are frames of the run: after six 2-substep frames with a
+    trial every second frame, the state is that of 12 plain substeps (within
+    float-atomic order) whatever CHUNK the tuner kept."""
+    from importlib import import_module
+
+    at = import_module("water-sim-cuda_amd.autotune")
+    b = small_batch
+    ref = batch_engine(gpu, b, RESORT_EVERY=2, CHUNK=256)
+    ref.steps(12)
+    expect = ref.getParticles()
+    ref.close()
+    e = batch_engine(gpu, b, RESORT_EVERY=2, CHUNK=256)
+    tuner = at.OnlineTuner(e, frame_substeps=2, period=2, chunks=(128, 256, 512))
+    for _ in range(6):
+        tuner.frame()
+    say(f"online tuner: {tuner.trials} trials, {tuner.switches} switches, CHUNK {e.get_option('CHUNK')}, "
+        f"history {tuner.history}")
+    assert tuner.frames == 6 and tuner.trials == 3 and len(tuner.history) == 3
+    assert e.get_option("CHUNK") in (128, 256, 512)
+    got = e.getParticles()
+    assert np.isfinite(got).all()
+    assert max_err(got[0:3], expect[0:3]) < 12 * POS_TOL
+    assert max_err(got[3:6], expect[3:6]) < 12 * VEL_TOL
+    e.close()

@@ -8,6 +8,12 @@ candidate, rewinding the state between trials (``Engine.snapshot`` /
 state as it was before tuning.  Results are cached in a JSON file keyed by
 the shape of the workload (scene sizes and particle counts), so later runs
 of the same workload skip the search.
+
+The fastest CHUNK can change as a run goes on: a chunk needs as many rounds
+as its fullest base cell holds particles, so where water compresses into
+dense cells, splitting blocks into more, smaller chunks gives the GPU more
+waves at no cost in rounds, while in the fresh state whole blocks win.
+``OnlineTuner`` therefore keeps tuning CHUNK during the run itself.
 """
 from __future__ import annotations
 
@@ -81,3 +87,75 @@ def autotune(eng, trial_substeps: int = 8, warmup: int = 2, chunks=CHUNKS, launc
         data[key] = {k: best[k] for k in ("CHUNK", "LAUNCH_PCT", "ms_per_substep")}
         path.write_text(json.dumps(data, indent=1, sort_keys=True))
     return result
+
+
+class OnlineTuner:
+    """Runs a caller's frames (``frame_substeps`` substeps each) and tunes
+    CHUNK on the way.  In every ``period`` frames, the second-to-last frame is
+    timed at the current CHUNK and the last one at a neighbouring value in
+    ``chunks`` (alternately the next larger and the next smaller); the
+    neighbour is kept when its frame was faster by more than ``margin``,
+    otherwise CHUNK goes back.  Nothing is rewound: the trial frames are
+    frames of the run, and a CHUNK change costs one full re-sort (run before
+    the frame's timer starts).  Only the timed frames synchronise with the
+    device."""
+
+    def __init__(self, eng, frame_substeps: int, period: int = 10, margin: float = 0.03, chunks=CHUNKS):
+        if frame_substeps < 1:
+            raise ValueError("frame_substeps must be at least 1")
+        self.eng = eng
+        self.k = int(frame_substeps)
+        self.period = max(2, int(period))
+        self.margin = float(margin)
+        self.chunks = tuple(sorted(set(int(c) for c in chunks)))
+        self.frames = 0
+        self.trials = 0
+        self.switches = 0
+        self.history = []  # (frame, CHUNK tried, its ms, the current CHUNK's ms, kept)
+        self._up = True
+        self._ref_ms = None
+
+    def _neighbour(self, cur: int):
+        lower = [c for c in self.chunks if c < cur]
+        higher = [c for c in self.chunks if c > cur]
+        first, second = (higher, lower) if self._up else (lower, higher)
+        self._up = not self._up
+        if first:
+            return first[0] if first is higher else first[-1]
+        if second:
+            return second[0] if second is higher else second[-1]
+        return None
+
+    def _timed(self) -> float:
+        e = self.eng
+        e.synchronize()
+        t0 = time.perf_counter()
+        e.steps(self.k)
+        e.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+
+    def frame(self) -> None:
+        """Runs one frame."""
+        e = self.eng
+        phase = self.frames % self.period
+        self.frames += 1
+        if phase == self.period - 2:
+            self._ref_ms = self._timed()
+            return
+        cand = None
+        if phase == self.period - 1 and self._ref_ms is not None:
+            cur = e.get_option("CHUNK")
+            cand = self._neighbour(cur)
+        if cand is None:
+            e.steps(self.k)
+            return
+        e.set_option("CHUNK", cand)
+        ms = self._timed()
+        self.trials += 1
+        kept = ms < self._ref_ms * (1.0 - self.margin)
+        if kept:
+            self.switches += 1
+        else:
+            e.set_option("CHUNK", cur)
+        self.history.append((self.frames - 1, cand, round(ms, 4), round(self._ref_ms, 4), kept))
+        self._ref_ms = None
