@@ -287,6 +287,10 @@ __device__ __forceinline__ uint32_t bucket_key(const DevCfg& c, const uint32_t* 
 // holds zeros on entry and the bucket sizes on exit.  The scene of slot i
 // comes from its caller index (orig, or i when orig is null).  A wave takes
 // BC_BATCH runs of 64 consecutive slots and issues their atomics together.
+// Every load of the wave's runs is issued before any is used (indices clamped
+// into [0, n), n > 0; lanes past n are masked at the atomics and stores), so
+// the wave waits once per dependent step (positions, block ranks, atomics)
+// rather than once per run and step.
 constexpr int BC_BATCH = 4;
 __global__ __launch_bounds__(256) void k_bucket_count(const float* __restrict__ px, const float* __restrict__ py,
                                                       const float* __restrict__ pz, int n, SceneTable t,
@@ -298,18 +302,29 @@ __global__ __launch_bounds__(256) void k_bucket_count(const float* __restrict__ 
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
     uint32_t key[BC_BATCH], b[BC_BATCH], got[BC_BATCH];
     uint64_t heads[BC_BATCH];
-    int nvalid[BC_BATCH];
+    int nvalid[BC_BATCH], at[BC_BATCH];
+    float x[BC_BATCH], y[BC_BATCH], z[BC_BATCH];
+#pragma unroll
+    for (int j = 0; j < BC_BATCH; ++j) {
+        at[j] = min((wave * BC_BATCH + j) * 64 + lane, n - 1);
+        x[j] = px[at[j]];
+        y[j] = py[at[j]];
+        z[j] = pz[at[j]];
+    }
+    if (t.nscenes > 1) {
+#pragma unroll
+        for (int j = 0; j < BC_BATCH; ++j)
+            key[j] = bucket_key(t.cfgs[scene_of(t, orig ? orig[at[j]] : at[j])], brank, x[j], y[j], z[j]);
+    } else {
+        const DevCfg c0 = uniform(t.cfgs[0]);
+#pragma unroll
+        for (int j = 0; j < BC_BATCH; ++j) key[j] = bucket_key(c0, brank, x[j], y[j], z[j]);
+    }
 #pragma unroll
     for (int j = 0; j < BC_BATCH; ++j) {
         const int i = (wave * BC_BATCH + j) * 64 + lane;
         const bool valid = i < n;  // the valid lanes of a batch are a prefix
-        key[j] = 0u;
-        b[j] = 0xFFFFFFFFu;
-        if (valid) {
-            const int sc = t.nscenes > 1 ? scene_of(t, orig ? orig[i] : i) : 0;
-            key[j] = bucket_key(t.cfgs[sc], brank, px[i], py[i], pz[i]);
-            b[j] = key[j] >> BLOCK_BITS;
-        }
+        b[j] = valid ? key[j] >> BLOCK_BITS : 0xFFFFFFFFu;
         const uint32_t prev = __shfl_up(b[j], 1);
         const bool head = valid && (lane == 0 || prev != b[j]);
         heads[j] = __ballot(head);
