@@ -314,19 +314,39 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
 
     const int lane = threadIdx.x;
     if constexpr (DO_G2P) {
-        for (int i = lane; i < TILE_N; i += TRANSFER_THREADS) {
-            int lx, ly, lz;
-            tile_coords(i, lx, ly, lz);
-            int x = d.ox + lx, y = d.oy + ly, z = d.oz + lz;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (inb(c, x, y, z)) {
-                int g = gidx(c, x, y, z);
-                v.x = a.vel[0][g];
-                v.y = a.vel[1][g];
-                v.z = a.vel[2][g];
-                v.w = a.sdf[g];
+        // All of the tile's loads are issued before any node is stored (one
+        // memory latency per chunk, not one per node of a lane): each node is
+        // read unconditionally at coordinates clamped into the grid (a guarded
+        // load would be waited for before the next one is issued), and nodes
+        // outside the grid are set to zero afterwards.  The node indices pass
+        // through an empty asm so that the compiler does not hoist their
+        // (lane-only) coordinates out of the chunk loop, which would hold them
+        // in registers through the rounds.
+        constexpr int TL_ITEMS = (TILE_N + TRANSFER_THREADS - 1) / TRANSFER_THREADS;
+        {
+            float4 tv[TL_ITEMS];
+#pragma unroll
+            for (int k = 0; k < TL_ITEMS; ++k) {
+                int i = min(lane + k * TRANSFER_THREADS, TILE_N - 1);
+                asm volatile("" : "+v"(i));
+                int lx, ly, lz;
+                tile_coords(i, lx, ly, lz);
+                const int x = d.ox + lx, y = d.oy + ly, z = d.oz + lz;
+                const int g =
+                    gidx(c, min(max(x, 0), c.sx - 1), min(max(y, 0), c.sy - 1), min(max(z, 0), c.sz - 1));
+                tv[k] = make_float4(a.vel[0][g], a.vel[1][g], a.vel[2][g], a.sdf[g]);
             }
-            vt[lx + TILE_PY * ly + TILE_PZ * lz] = v;
+#pragma unroll
+            for (int k = 0; k < TL_ITEMS; ++k) {
+                int i = lane + k * TRANSFER_THREADS;
+                asm volatile("" : "+v"(i));
+                if (i < TILE_N) {
+                    int lx, ly, lz;
+                    tile_coords(i, lx, ly, lz);
+                    const bool in = inb(c, d.ox + lx, d.oy + ly, d.oz + lz);
+                    vt[lx + TILE_PY * ly + TILE_PZ * lz] = in ? tv[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+            }
         }
     }
     if constexpr (DO_P2G) {
