@@ -462,10 +462,11 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
     static_assert(BLOCK_CELLS_N == 128 && RO_THREADS >= 128, "one wave scans the cell counts, two per lane");
     extern __shared__ uint32_t ro_lds[];
     uint32_t* cell_rank = ro_lds;          // [cap] cell | atomic rank << BLOCK_BITS per chunk position;
-                                           //       then the list in final order (position | cell << 16)
+                                           //       then the caller indices in list order; then the new
+                                           //       slot (from start) of each chunk position
     uint32_t* org = ro_lds + cap;          // [cap] caller index per chunk position
     uint32_t* lst = ro_lds + 2 * cap;      // [cap] list in atomic order (position | cell << 16);
-                                           //       then the new slot (from start) of each chunk position
+                                           //       then in final order
     __shared__ uint32_t cnt[BLOCK_CELLS_N];  // zero between chunks
     __shared__ uint32_t cfirst[BLOCK_CELLS_N];
     __shared__ uint32_t cmax;
@@ -547,18 +548,36 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
             lst[cfirst[c] + (v >> BLOCK_BITS)] = (uint32_t)q | (c << 16);
         }
         __syncthreads();
-        // each list entry's rank among its cell's members by caller index
-        for (int p = tid; p < count; p += RO_THREADS) {
-            const uint32_t e = lst[p], q = e & 0xFFFFu, c = e >> 16;
-            const uint32_t first = cfirst[c], size = cnt[c];
-            uint32_t rank = (uint32_t)p - first;
-            if (size <= (uint32_t)RO_SORT_MAX) {
-                const uint32_t g = org[q];
-                rank = 0u;
-#pragma unroll 4
-                for (uint32_t k = 0; k < size; ++k) rank += org[lst[first + k] & 0xFFFFu] < g ? 1u : 0u;
+        // each list entry's rank among its cell's members by caller index: the
+        // entries are taken into registers and their caller indices copied
+        // into list order, so that the comparisons read one array; the
+        // entries then go back to the list in their final order
+        uint32_t ent[RO_ITEMS];
+#pragma unroll
+        for (int i = 0; i < RO_ITEMS; ++i) {
+            const int p = tid + RO_THREADS * i;
+            ent[i] = 0u;
+            if (p < count) {
+                ent[i] = lst[p];
+                cell_rank[p] = org[ent[i] & 0xFFFFu];
             }
-            cell_rank[first + rank] = q | (c << 16);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < RO_ITEMS; ++i) {
+            const int p = tid + RO_THREADS * i;
+            if (p < count) {
+                const uint32_t c = ent[i] >> 16;
+                const uint32_t first = cfirst[c], size = cnt[c];
+                uint32_t rank = (uint32_t)p - first;
+                if (size <= (uint32_t)RO_SORT_MAX) {
+                    const uint32_t g = cell_rank[p];
+                    rank = 0u;
+#pragma unroll 4
+                    for (uint32_t k = 0; k < size; ++k) rank += cell_rank[first + k] < g ? 1u : 0u;
+                }
+                lst[first + rank] = ent[i];
+            }
         }
         load_orig();  // the next chunk's slots have arrived by now
         __syncthreads();
@@ -570,7 +589,7 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
         for (uint32_t r = (uint32_t)wave; r < S; r += RO_THREADS / 64) {
             const uint32_t m = r < full ? nl : nl - 1u;
             const bool act = (uint32_t)lane < m;
-            const uint32_t e = act ? cell_rank[(uint32_t)lane * S + r] : 0u;
+            const uint32_t e = act ? lst[(uint32_t)lane * S + r] : 0u;
             const uint32_t c = e >> 16;
             const uint32_t cx = ((c >> 1) & 1u) | (((c >> 4) & 1u) << 1) | ((c & 1u) << 2);
             const uint32_t cy = ((c >> 2) & 1u) | (((c >> 5) & 1u) << 1);
@@ -587,7 +606,7 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
             uint32_t pos = 0u;
 #pragma unroll
             for (uint32_t h = 0; h < 8; ++h) pos += min(cg[h], rank) + ((h < g && cg[h] > rank) ? 1u : 0u);
-            if (act) lst[e & 0xFFFFu] = (r < full ? r * nl : full * nl + (r - full) * (nl - 1u)) + pos;
+            if (act) cell_rank[e & 0xFFFFu] = (r < full ? r * nl : full * nl + (r - full) * (nl - 1u)) + pos;
         }
         if (tid == 0) {
             ch[ci].S = (int)S;
@@ -598,7 +617,7 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
         for (int i = 0; i < RO_ITEMS; ++i) {
             const int q = tid + RO_THREADS * i;
             if (q < count) {
-                const size_t t = (size_t)start + lst[q];
+                const size_t t = (size_t)start + cell_rank[q];
                 map[t] = si[i];
                 if constexpr (PLANES) {
 #pragma unroll
