@@ -196,10 +196,13 @@ def test_autotune_keeps_state_and_caches(gpu, small_batch, tmp_path):
     e.close()
 
 
-def test_online_tuner_runs_the_frames(gpu, small_batch):
+@pytest.mark.parametrize("graph", [0, 1])
+def test_online_tuner_runs_the_frames(gpu, small_batch, graph):
     """Trial frames are frames of the run: after six 2-substep frames with a
     trial every second frame, the state is that of 12 plain substeps (within
-    float-atomic order) whatever CHUNK the tuner kept."""
+    float-atomic order) whatever CHUNK the tuner kept.  With graph replay, a
+    CHUNK change drops the captured frame graph and the next frame captures
+    a new one."""
     from importlib import import_module
 
     at = import_module("water-sim-cuda_amd.autotune")
@@ -208,12 +211,12 @@ def test_online_tuner_runs_the_frames(gpu, small_batch):
     ref.steps(12)
     expect = ref.getParticles()
     ref.close()
-    e = batch_engine(gpu, b, RESORT_EVERY=2, CHUNK=256)
+    e = batch_engine(gpu, b, RESORT_EVERY=2, CHUNK=256, GRAPH=graph)
     tuner = at.OnlineTuner(e, frame_substeps=2, period=2, chunks=(128, 256, 512))
     for _ in range(6):
         tuner.frame()
-    say(f"online tuner: {tuner.trials} trials, {tuner.switches} switches, CHUNK {e.get_option('CHUNK')}, "
-        f"history {tuner.history}")
+    say(f"online tuner (graph {graph}): {tuner.trials} trials, {tuner.switches} switches, "
+        f"CHUNK {e.get_option('CHUNK')}, history {tuner.history}")
     assert tuner.frames == 6 and tuner.trials == 3 and len(tuner.history) == 3
     assert e.get_option("CHUNK") in (128, 256, 512)
     got = e.getParticles()

@@ -98,7 +98,9 @@ class OnlineTuner:
     otherwise CHUNK goes back.  Nothing is rewound: the trial frames are
     frames of the run, and a CHUNK change costs one full re-sort (run before
     the frame's timer starts).  Only the timed frames synchronise with the
-    device."""
+    device.  With graph replay a CHUNK change makes the next frame capture a
+    new graph; with ``period`` of 3 or more that frame is not the reference
+    frame, whose time would otherwise include the capture."""
 
     def __init__(self, eng, frame_substeps: int, period: int = 10, margin: float = 0.03, chunks=CHUNKS):
         if frame_substeps < 1:
