@@ -125,7 +125,9 @@ int mpm_engine_destroy(mpm_engine* h);
  *                   same field as the brute-force makeSDF (kernels.cu:282-300,
  *                   run with 0): equal distances, equal signs wherever the
  *                   distance is non-zero
- * Read-only (get_option): "GRAPH_REPLAYS" graph launches so far, "SCENES"
+ * Read-only (get_option): "GRAPH_REPLAYS" graph launches so far,
+ * "GRAPH_CAPTURES" graphs captured so far (a steps(k) call whose schedule
+ * state matched no cached graph), "SCENES"
  * scene count, "SDF_EXACT_NODES" nodes of the last hierarchy initSDF whose
  * sign took the exact winding sum. */
 int mpm_engine_set_option(mpm_engine* h, const char* key, int64_t value);

@@ -173,8 +173,8 @@ void oracle_init_state_f64(const Config* c, int n, double* soa) { init_state<dou
 void oracle_steps_f32(const Config* c, int n, float* soa, const float* sdf, int nsteps, int nthreads) {
     run_steps<float>(*c, n, soa, sdf, nsteps, nthreads);
 }
-void oracle_steps_f64(const Config* c, int n, double* soa, const double* sdf, int nsteps) {
-    run_steps<double>(*c, n, soa, sdf, nsteps, 1);
+void oracle_steps_f64(const Config* c, int n, double* soa, const double* sdf, int nsteps, int nthreads) {
+    run_steps<double>(*c, n, soa, sdf, nsteps, nthreads);
 }
 
 // Stage entry points.  grid4 = 4*G values (mass, mom_x, mom_y, mom_z planes).

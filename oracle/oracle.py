@@ -55,7 +55,7 @@ def lib() -> ctypes.CDLL:
             "oracle_init_state_f32": (None, [cp, ctypes.c_int, f]),
             "oracle_init_state_f64": (None, [cp, ctypes.c_int, d]),
             "oracle_steps_f32": (None, [cp, ctypes.c_int, f, f, ctypes.c_int, ctypes.c_int]),
-            "oracle_steps_f64": (None, [cp, ctypes.c_int, d, d, ctypes.c_int]),
+            "oracle_steps_f64": (None, [cp, ctypes.c_int, d, d, ctypes.c_int, ctypes.c_int]),
             "oracle_p2g_f32": (None, [cp, ctypes.c_int, f, i32, f]),
             "oracle_p2g_f64": (None, [cp, ctypes.c_int, d, i32, d]),
             "oracle_grid_update_f32": (None, [cp, f, f]),
@@ -120,7 +120,7 @@ def steps(cfg, state: np.ndarray, sdf: np.ndarray, k: int, nthreads: int = 1) ->
     if state.dtype == np.float64:
         p = np.ascontiguousarray(state, dtype=np.float64).copy()
         s = np.ascontiguousarray(sdf, dtype=np.float64)
-        lib().oracle_steps_f64(ctypes.byref(c), n, _d(p), _d(s), k)
+        lib().oracle_steps_f64(ctypes.byref(c), n, _d(p), _d(s), k, nthreads)
     else:
         p = np.ascontiguousarray(state, dtype=np.float32).copy()
         s = np.ascontiguousarray(sdf, dtype=np.float32)

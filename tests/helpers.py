@@ -63,3 +63,34 @@ def signed_volume(tris: np.ndarray) -> float:
     """Divergence theorem: sum of signed tetra volumes (origin, v0, v1, v2)."""
     t = tris.astype(np.float64).reshape(-1, 3, 3)
     return float(np.einsum("ij,ij->i", t[:, 0], np.cross(t[:, 1], t[:, 2])).sum() / 6.0)
+
+
+def host_threads() -> int:
+    """Host CPUs this process may use (affinity set, capped by a cgroup CPU
+    quota), for the OpenMP oracle."""
+    import os
+    from pathlib import Path
+
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    try:
+        quota, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, int(quota) // int(period)))
+    except (OSError, ValueError):
+        pass
+    return max(1, n)
+
+
+def parity_log(record: dict) -> None:
+    """Appends a JSON line to $MPM_PARITY_LOG when it is set (the per-config
+    error table of the full-size runs)."""
+    import json
+    import os
+
+    path = os.environ.get("MPM_PARITY_LOG")
+    if path:
+        with open(path, "a") as f:
+            f.write(json.dumps(record) + "\n")

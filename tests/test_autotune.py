@@ -7,13 +7,17 @@ import pytest
 
 class StubEngine:
     """steps(k) advances a fake clock by k times the per-substep cost of the
-    current CHUNK."""
+    current CHUNK.  It models a one-entry graph cache: a steps() call whose
+    CHUNK differs from the previous call's "captures" a graph."""
 
     def __init__(self, cost, chunk):
         self.cost, self.chunk = cost, chunk
         self.substeps, self.clock, self.sets = 0, 0.0, []
+        self.captures, self._graph_chunk = 0, None
 
     def get_option(self, key):
+        if key == "GRAPH_CAPTURES":
+            return self.captures
         assert key == "CHUNK"
         return self.chunk
 
@@ -26,6 +30,9 @@ class StubEngine:
         pass
 
     def steps(self, k):
+        if self._graph_chunk != self.chunk:
+            self.captures += 1
+            self._graph_chunk = self.chunk
         self.substeps += k
         self.clock += self.cost[self.chunk] * k
 
@@ -35,7 +42,7 @@ def at(monkeypatch):
     return import_module("water-sim-cuda_amd.autotune")
 
 
-def run(at, monkeypatch, cost, start, frames, period=2, chunks=(128, 256, 512, 1024)):
+def run(at, monkeypatch, cost, start, frames, period=4, chunks=(128, 256, 512, 1024)):
     e = StubEngine(cost, start)
     monkeypatch.setattr(at.time, "perf_counter", lambda: e.clock)
     t = at.OnlineTuner(e, 4, period=period, chunks=chunks)
@@ -46,24 +53,35 @@ def run(at, monkeypatch, cost, start, frames, period=2, chunks=(128, 256, 512, 1
 
 def test_keeps_a_faster_neighbour_and_reverts_slower_ones(at, monkeypatch):
     cost = {128: 3.0, 256: 2.0, 512: 1.0, 1024: 1.5}
-    e, t = run(at, monkeypatch, cost, 256, 8)
-    # frame 1 tries 512 (kept), 3 tries 256, 5 tries 1024, 7 tries 256 (all reverted)
-    assert e.substeps == 32 and t.frames == 8 and t.trials == 4 and t.switches == 1
+    e, t = run(at, monkeypatch, cost, 256, 16)
+    # frame 3 times 512 (kept), 7 times 256, 11 times 1024, 15 times 256 (all reverted)
+    assert e.substeps == 64 and t.frames == 16 and t.trials == 4 and t.switches == 1
     assert e.chunk == 512
+    assert [h[0] for h in t.history] == [3, 7, 11, 15]
     assert [h[1] for h in t.history] == [512, 256, 1024, 256]
     assert [h[4] for h in t.history] == [True, False, False, False]
     assert e.sets == [512, 256, 512, 1024, 512, 256, 512]
 
 
+def test_timed_frames_never_include_a_capture(at, monkeypatch):
+    """The frame after a CHUNK change (which captures the new graph) is not
+    timed: neither timed frame of any trial captured."""
+    cost = {128: 3.0, 256: 2.0, 512: 1.0, 1024: 1.5}
+    e, t = run(at, monkeypatch, cost, 256, 40)
+    assert t.trials == 10
+    assert not any(h[5] for h in t.history)
+    assert e.captures > t.trials  # every switch and every revert captured, untimed
+
+
 def test_margin_keeps_the_current_chunk_on_a_tie(at, monkeypatch):
     cost = {128: 1.0, 256: 0.99, 512: 1.0, 1024: 1.0}  # 1 % faster: inside the 3 % margin
-    e, t = run(at, monkeypatch, cost, 512, 6)
+    e, t = run(at, monkeypatch, cost, 512, 12)
     assert e.chunk == 512 and t.switches == 0 and t.trials == 3
 
 
 def test_ends_of_the_list_try_the_only_neighbour(at, monkeypatch):
     cost = {128: 1.0, 256: 2.0}
-    e, t = run(at, monkeypatch, cost, 128, 4, chunks=(128, 256))
+    e, t = run(at, monkeypatch, cost, 128, 8, chunks=(128, 256))
     assert [h[1] for h in t.history] == [256, 256] and e.chunk == 128
 
 
@@ -74,6 +92,51 @@ def test_period_schedules_one_trial_per_period(at, monkeypatch):
     assert [h[0] for h in t.history] == [9, 19, 29]
 
 
+def test_short_periods_are_raised_to_four(at):
+    assert at.OnlineTuner(StubEngine({256: 1.0}, 256), 4, period=2).period == 4
+
+
 def test_rejects_empty_frames(at):
     with pytest.raises(ValueError):
         at.OnlineTuner(StubEngine({}, 256), 0)
+
+
+class SnapEngine(StubEngine):
+    """StubEngine with the snapshot/restore and LAUNCH_PCT the offline
+    autotuner uses; restore() returns to the snapshot's CHUNK graph state."""
+
+    def __init__(self, cost, chunk):
+        super().__init__(cost, chunk)
+        self.pct = 100
+        self.cfg = type("Cfg", (), {"SIZE_X": 32, "SIZE_Y": 32, "SIZE_Z": 32})()
+
+    def get_option(self, key):
+        if key == "LAUNCH_PCT":
+            return self.pct
+        return super().get_option(key)
+
+    def set_option(self, key, value):
+        if key == "LAUNCH_PCT":
+            self.pct = value
+            return
+        super().set_option(key, value)
+
+    def getNumber(self):
+        return 1000
+
+    def snapshot(self):
+        pass
+
+    def restore(self):
+        pass
+
+
+def test_autotune_trials_time_replays_not_captures(at, monkeypatch):
+    cost = {128: 3.0, 256: 2.0, 512: 1.0, 1024: 1.5, 2048: 1.2}
+    e = SnapEngine(cost, 2048)
+    monkeypatch.setattr(at.time, "perf_counter", lambda: e.clock)
+    r = at.autotune(e, trial_substeps=8, warmup=2, launch_pcts=(100,))
+    assert r["CHUNK"] == 512 and not r["cached"]
+    assert len(r["trials"]) == 5
+    assert not any(t["captured"] for t in r["trials"])
+    assert [t["ms_per_substep"] for t in r["trials"]] == [3.0e3, 2.0e3, 1.0e3, 1.5e3, 1.2e3]

@@ -198,29 +198,60 @@ def test_autotune_keeps_state_and_caches(gpu, small_batch, tmp_path):
 
 @pytest.mark.parametrize("graph", [0, 1])
 def test_online_tuner_runs_the_frames(gpu, small_batch, graph):
-    """Trial frames are frames of the run: after six 2-substep frames with a
-    trial every second frame, the state is that of 12 plain substeps (within
-    float-atomic order) whatever CHUNK the tuner kept.  With graph replay, a
-    CHUNK change drops the captured frame graph and the next frame captures
-    a new one."""
+    """Trial frames are frames of the run: after twelve 2-substep frames with
+    a trial in every fourth frame, the state is that of 24 plain substeps
+    (within float-atomic order) whatever CHUNK the tuner kept.  With graph
+    replay, frames replay their graph, a CHUNK change makes the next
+    (untimed) frame capture a new one, and no timed frame captures."""
     from importlib import import_module
 
     at = import_module("water-sim-cuda_amd.autotune")
     b = small_batch
     ref = batch_engine(gpu, b, RESORT_EVERY=2, CHUNK=256)
-    ref.steps(12)
+    ref.steps(24)
     expect = ref.getParticles()
     ref.close()
     e = batch_engine(gpu, b, RESORT_EVERY=2, CHUNK=256, GRAPH=graph)
-    tuner = at.OnlineTuner(e, frame_substeps=2, period=2, chunks=(128, 256, 512))
-    for _ in range(6):
+    tuner = at.OnlineTuner(e, frame_substeps=2, period=4, chunks=(128, 256, 512))
+    for _ in range(12):
         tuner.frame()
     say(f"online tuner (graph {graph}): {tuner.trials} trials, {tuner.switches} switches, "
-        f"CHUNK {e.get_option('CHUNK')}, history {tuner.history}")
-    assert tuner.frames == 6 and tuner.trials == 3 and len(tuner.history) == 3
+        f"CHUNK {e.get_option('CHUNK')}, history {tuner.history}, replays {e.get_option('GRAPH_REPLAYS')}, "
+        f"captures {e.get_option('GRAPH_CAPTURES')}")
+    assert tuner.frames == 12 and tuner.trials == 3 and len(tuner.history) == 3
     assert e.get_option("CHUNK") in (128, 256, 512)
+    assert not any(h[5] for h in tuner.history), "a timed frame captured a graph"
+    if graph:
+        assert e.get_option("GRAPH_REPLAYS") == 12
+        assert e.get_option("GRAPH_CAPTURES") < 12
     got = e.getParticles()
     assert np.isfinite(got).all()
-    assert max_err(got[0:3], expect[0:3]) < 12 * POS_TOL
-    assert max_err(got[3:6], expect[3:6]) < 12 * VEL_TOL
+    assert max_err(got[0:3], expect[0:3]) < 24 * POS_TOL
+    assert max_err(got[3:6], expect[3:6]) < 24 * VEL_TOL
     e.close()
+
+
+def test_resort_every_change_drops_the_graph(gpu, small_batch):
+    """Changing RESORT_EVERY under graph replay takes effect: the next call
+    captures a new graph, and the state follows an eager engine run with the
+    same schedule (within float-atomic order)."""
+    b = small_batch
+    eager = batch_engine(gpu, b, RESORT_EVERY=4, CHUNK=256)
+    e = batch_engine(gpu, b, RESORT_EVERY=4, CHUNK=256, GRAPH=1)
+    for eng in (eager, e):
+        for _ in range(3):  # the re-sort phase at a call's start settles by the third call
+            eng.steps(4)
+    c0 = e.get_option("GRAPH_CAPTURES")
+    e.steps(4)
+    assert e.get_option("GRAPH_CAPTURES") == c0  # same schedule: replay
+    eager.steps(4)
+    for eng in (eager, e):
+        eng.set_option("RESORT_EVERY", 1)
+    e.steps(4)
+    assert e.get_option("GRAPH_CAPTURES") == c0 + 1  # new schedule: new graph
+    eager.steps(4)
+    got, expect = e.getParticles(), eager.getParticles()
+    assert max_err(got[0:3], expect[0:3]) < 20 * POS_TOL
+    assert max_err(got[3:6], expect[3:6]) < 20 * VEL_TOL
+    e.close()
+    eager.close()

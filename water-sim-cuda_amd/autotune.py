@@ -49,9 +49,34 @@ def _load(path: Path) -> dict:
         return {}
 
 
+def _captures(eng) -> int:
+    """Graphs the engine has captured so far (0 for engines without graphs)."""
+    try:
+        return int(eng.get_option("GRAPH_CAPTURES"))
+    except Exception:
+        return 0
+
+
+def _settle(eng, k: int, tries: int = 3) -> None:
+    """Runs untimed steps(k) calls until one of them replays a cached graph
+    instead of capturing a new one (at most `tries` calls; one call without
+    graph replay), so that a following timed steps(k) call measures a replay
+    and not a capture."""
+    for _ in range(tries):
+        before = _captures(eng)
+        eng.steps(k)
+        if _captures(eng) == before:
+            break
+
+
 def autotune(eng, trial_substeps: int = 8, warmup: int = 2, chunks=CHUNKS, launch_pcts=LAUNCH_PCTS,
              cache: str | os.PathLike | None = None) -> dict:
-    """Tunes ``eng`` in place and returns {"CHUNK", "LAUNCH_PCT", "ms_per_substep", "trials", "cached"}."""
+    """Tunes ``eng`` in place and returns {"CHUNK", "LAUNCH_PCT", "ms_per_substep", "trials", "cached"}.
+
+    Each trial sets the candidate, runs untimed steps(trial_substeps) calls
+    (at least ``warmup`` substeps, and with graph replay until a call replays
+    its graph), then times one steps(trial_substeps) call; a trial whose
+    timed call still captured a graph is marked ``captured``."""
     key = signature(eng)
     path = Path(cache) if cache else None
     if path is not None:
@@ -69,13 +94,17 @@ def autotune(eng, trial_substeps: int = 8, warmup: int = 2, chunks=CHUNKS, launc
         for pct in launch_pcts:
             eng.set_option("CHUNK", ch)
             eng.set_option("LAUNCH_PCT", pct)
-            eng.steps(warmup)
+            for _ in range(max(1, -(-warmup // trial_substeps))):
+                eng.steps(trial_substeps)
+            _settle(eng, trial_substeps)
             eng.synchronize()
+            c0 = _captures(eng)
             t0 = time.perf_counter()
             eng.steps(trial_substeps)
             eng.synchronize()
             ms = (time.perf_counter() - t0) * 1e3 / trial_substeps
-            trials.append({"CHUNK": ch, "LAUNCH_PCT": pct, "ms_per_substep": round(ms, 5)})
+            trials.append({"CHUNK": ch, "LAUNCH_PCT": pct, "ms_per_substep": round(ms, 5),
+                           "captured": _captures(eng) != c0})
             eng.restore()
     best = min(trials, key=lambda t: t["ms_per_substep"])
     eng.set_option("CHUNK", best["CHUNK"])
@@ -91,31 +120,36 @@ def autotune(eng, trial_substeps: int = 8, warmup: int = 2, chunks=CHUNKS, launc
 
 class OnlineTuner:
     """Runs a caller's frames (``frame_substeps`` substeps each) and tunes
-    CHUNK on the way.  In every ``period`` frames, the second-to-last frame is
-    timed at the current CHUNK and the last one at a neighbouring value in
-    ``chunks`` (alternately the next larger and the next smaller); the
-    neighbour is kept when its frame was faster by more than ``margin``,
-    otherwise CHUNK goes back.  Nothing is rewound: the trial frames are
-    frames of the run, and a CHUNK change costs one full re-sort (run before
-    the frame's timer starts).  Only the timed frames synchronise with the
-    device.  With graph replay a CHUNK change makes the next frame capture a
-    new graph; with ``period`` of 3 or more that frame is not the reference
-    frame, whose time would otherwise include the capture."""
+    CHUNK on the way.  In every ``period`` frames (at least 4): frame
+    period-3 is timed at the current CHUNK; at frame period-2 CHUNK moves to
+    a neighbouring value in ``chunks`` (alternately the next larger and the
+    next smaller), and that frame runs untimed, taking the change's re-sort,
+    the new layout's first launches and, with graph replay, the capture of
+    the new frame graph; frame period-1 is timed at the neighbour, which is
+    kept when it was faster by more than ``margin``, otherwise CHUNK goes
+    back.  Nothing is rewound: every frame is a frame of the run.  Only the
+    timed frames synchronise with the device, and ``history`` records for
+    each trial whether either timed frame captured a graph (it should not:
+    the frames before them ran the same schedule)."""
 
     def __init__(self, eng, frame_substeps: int, period: int = 10, margin: float = 0.03, chunks=CHUNKS):
         if frame_substeps < 1:
             raise ValueError("frame_substeps must be at least 1")
         self.eng = eng
         self.k = int(frame_substeps)
-        self.period = max(2, int(period))
+        self.period = max(4, int(period))
         self.margin = float(margin)
         self.chunks = tuple(sorted(set(int(c) for c in chunks)))
         self.frames = 0
         self.trials = 0
         self.switches = 0
-        self.history = []  # (frame, CHUNK tried, its ms, the current CHUNK's ms, kept)
+        # (frame, CHUNK tried, its ms, the current CHUNK's ms, kept, a timed frame captured a graph)
+        self.history = []
         self._up = True
         self._ref_ms = None
+        self._ref_cap = False
+        self._cur = None
+        self._cand = None
 
     def _neighbour(self, cur: int):
         lower = [c for c in self.chunks if c < cur]
@@ -128,36 +162,41 @@ class OnlineTuner:
             return second[0] if second is higher else second[-1]
         return None
 
-    def _timed(self) -> float:
+    def _timed(self):
         e = self.eng
         e.synchronize()
+        c0 = _captures(e)
         t0 = time.perf_counter()
         e.steps(self.k)
         e.synchronize()
-        return (time.perf_counter() - t0) * 1e3
+        return (time.perf_counter() - t0) * 1e3, _captures(e) != c0
 
     def frame(self) -> None:
         """Runs one frame."""
         e = self.eng
         phase = self.frames % self.period
         self.frames += 1
-        if phase == self.period - 2:
-            self._ref_ms = self._timed()
+        if phase == self.period - 3:
+            self._ref_ms, self._ref_cap = self._timed()
             return
-        cand = None
-        if phase == self.period - 1 and self._ref_ms is not None:
-            cur = e.get_option("CHUNK")
-            cand = self._neighbour(cur)
-        if cand is None:
+        if phase == self.period - 2 and self._ref_ms is not None:
+            self._cur = e.get_option("CHUNK")
+            self._cand = self._neighbour(self._cur)
+            if self._cand is not None:
+                e.set_option("CHUNK", self._cand)
             e.steps(self.k)
             return
-        e.set_option("CHUNK", cand)
-        ms = self._timed()
-        self.trials += 1
-        kept = ms < self._ref_ms * (1.0 - self.margin)
-        if kept:
-            self.switches += 1
-        else:
-            e.set_option("CHUNK", cur)
-        self.history.append((self.frames - 1, cand, round(ms, 4), round(self._ref_ms, 4), kept))
-        self._ref_ms = None
+        if phase == self.period - 1 and self._cand is not None:
+            ms, cap = self._timed()
+            self.trials += 1
+            kept = ms < self._ref_ms * (1.0 - self.margin)
+            if kept:
+                self.switches += 1
+            else:
+                e.set_option("CHUNK", self._cur)
+            self.history.append((self.frames - 1, self._cand, round(ms, 4), round(self._ref_ms, 4), kept,
+                                 bool(cap or self._ref_cap)))
+            self._ref_ms = None
+            self._cand = None
+            return
+        e.steps(self.k)

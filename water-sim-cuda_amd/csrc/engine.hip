@@ -124,6 +124,7 @@ struct mpm_engine {
         int end_vcur = 0, end_pcur = 0;
     } graph;
     int64_t graph_replays = 0;
+    int64_t graph_captures = 0;
 
     // Learned corrector: applied after every grid update while `correct`.
     const CorrState* corr = nullptr;
@@ -511,6 +512,7 @@ int steps_graph(mpm_engine* h, int k) {
         h->pcur = e.pcur;
         if (ie != hipSuccess) return fail(MPM_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
         g = e;
+        ++h->graph_captures;
     }
     HIPCHK(hipGraphLaunch(g.exec, h->stream));
     ++h->graph_replays;
@@ -834,7 +836,10 @@ int mpm_engine_set_option(mpm_engine* h, const char* key, int64_t value) {
     std::string k(key);
     if (k == "RESORT_EVERY") {
         if (value < 1) return fail(MPM_EINVAL, "RESORT_EVERY must be >= 1");
-        h->resort_every = (int)std::min<int64_t>(value, 1 << 30);
+        const int re = (int)std::min<int64_t>(value, 1 << 30);
+        // a captured steps(k) graph bakes the re-sort schedule in
+        if (re != h->resort_every) drop_graph(h);
+        h->resort_every = re;
     } else if (k == "CHUNK") {
         if (value < 64 || value > CHUNK_CAP_MAX || (value % 64) != 0)
             return fail(MPM_EINVAL, "CHUNK must be a multiple of 64 in [64, 2048]");
@@ -885,6 +890,7 @@ int mpm_engine_get_option(mpm_engine* h, const char* key, int64_t* value) {
     else if (k == "GRAPH") *value = h->use_graph ? 1 : 0;
     else if (k == "CORRECT") *value = h->correct ? 1 : 0;
     else if (k == "GRAPH_REPLAYS") *value = h->graph_replays;
+    else if (k == "GRAPH_CAPTURES") *value = h->graph_captures;
     else if (k == "SCENES") *value = h->S;
     else return fail(MPM_EINVAL, "unknown option " + k);
     return MPM_OK;

@@ -38,6 +38,30 @@ def _pkg():
     return importlib.import_module("water-sim-cuda_amd")
 
 
+# Parameter blocks of the network in the order of the flat parameter vector
+# (W[in][out] row-major, then the bias), as mpm_corrector_set_params reads it.
+LAYERS = ((66, 64), (64, 64), (64, 64), (64, 3))
+
+
+def init_params(seed: int = 0, last_gain: float = 0.1) -> np.ndarray:
+    """Seeded He-uniform initialisation: W ~ U(-a, a) with a = sqrt(6 / fan_in)
+    (He et al. 2015, the uniform form of their ReLU variance 2 / fan_in),
+    biases zero; the output layer is scaled by `last_gain` so that the
+    untrained network's corrections start near zero.  Float32, flat in the
+    layout of LAYERS."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for li, (fan_in, fan_out) in enumerate(LAYERS):
+        a = np.sqrt(6.0 / fan_in)
+        if li == len(LAYERS) - 1:
+            a *= last_gain
+        out.append(rng.uniform(-a, a, size=(fan_in, fan_out)).reshape(-1))
+        out.append(np.zeros(fan_out))
+    p = np.concatenate(out).astype(np.float32)
+    assert p.size == 12803
+    return p
+
+
 def dambreak_family(seed: int, physical: bool = True) -> dict:
     """Member `seed` of the 128^3 dam-break family: the 48x96x48 block of the
     base scene shifted along z and resized by up to 8 cells, jitter seeded,
