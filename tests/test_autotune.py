@@ -73,6 +73,44 @@ def test_timed_frames_never_include_a_capture(at, monkeypatch):
     assert e.captures > t.trials  # every switch and every revert captured, untimed
 
 
+class AlternatingStub(StubEngine):
+    """A graph cache keyed by (CHUNK, call parity): the engine's start state
+    alternates between calls, so a new CHUNK captures on two calls in a row;
+    a CHUNK change drops every cached graph."""
+
+    def __init__(self, cost, chunk):
+        super().__init__(cost, chunk)
+        self.cache, self.calls = set(), 0
+
+    def set_option(self, key, value):
+        super().set_option(key, value)
+        self.cache.clear()
+
+    def steps(self, k):
+        key = (self.chunk, self.calls % 2)
+        if key not in self.cache:
+            self.captures += 1
+            self.cache.add(key)
+        self.calls += 1
+        self.substeps += k
+        self.clock += self.cost[self.chunk] * k
+
+
+def test_timed_frame_after_a_capture_is_retried(at, monkeypatch):
+    cost = {128: 3.0, 256: 2.0, 512: 1.0, 1024: 1.5}
+    e = AlternatingStub(cost, 256)
+    monkeypatch.setattr(at.time, "perf_counter", lambda: e.clock)
+    t = at.OnlineTuner(e, 4, period=6, chunks=(128, 256, 512, 1024))
+    for _ in range(30):
+        t.frame()
+    assert t.frames == 30 and t.trials >= 3
+    # each candidate frame after the untimed switch frame captured the other
+    # parity's graph once and was timed again on the next frame
+    assert all(h[5] >= 1 for h in t.history)
+    assert t.history[0][1] == 512 and t.history[0][4]
+    assert e.chunk == 512
+
+
 def test_margin_keeps_the_current_chunk_on_a_tie(at, monkeypatch):
     cost = {128: 1.0, 256: 0.99, 512: 1.0, 1024: 1.0}  # 1 % faster: inside the 3 % margin
     e, t = run(at, monkeypatch, cost, 512, 12)

@@ -189,3 +189,43 @@ def test_scene_from_config(mpm, tmp_path):
     assert not parts[3:24].any() and (parts[24:26] == 1.0).all()
     np.testing.assert_array_equal(tris, mpm.obj_collider(tmp_path / "cube.obj", 64.0, 20.0, (0.0, -20.0, 0.0)))
     assert cfg.SIZE_X == 64 and abs(cfg.DT - 0.001) < 1e-9
+
+
+def test_reference_scene_fixture(mpm):
+    """tests/golden/reference_scene.npz holds the reference program's own
+    scene (config.txt:1-18 with models/sphere.obj twice and models/u.obj):
+    one particle per lattice node of the two dilated sphere shells, merged
+    without de-duplication (Simulation::MergeVoxelData, Simulation.cpp:122-160),
+    VOL = MASS = 1 and zero elsewhere (VoxelPipeline.cpp:9-30), and the U's
+    quads split into triangles inside the grid.  Where the reference tree is
+    present, the front-end rebuilds it bit for bit."""
+    from pathlib import Path
+
+    golden = Path(__file__).resolve().parent / "golden"
+    d = np.load(golden / "reference_scene.npz")
+    cfg, parts, tris = d["cfg"], d["particles"], d["triangles"]
+    assert list(d["cfg_fields"][:3]) == ["SIZE_X", "SIZE_Y", "SIZE_Z"] and list(cfg[:4]) == [128, 128, 128, 2]
+    n = parts.shape[1]
+    assert parts.shape == (26, n) and n > 5000
+    assert np.array_equal(parts[0:3], np.round(parts[0:3]))  # RESOLUTION 1: integer lattice
+    assert (parts[24] == 1).all() and (parts[25] == 1).all() and not parts[3:24].any()
+    # two copies of one sphere shell, the second shifted by (10, 0, 0): equal halves
+    half = n // 2
+    assert n == 2 * half
+    a, b = parts[0:3, :half], parts[0:3, half:]
+    np.testing.assert_array_equal(np.sort(b[0] - 10.0), np.sort(a[0]))
+    assert tris.shape[1] == 9 and 0 < tris.shape[0] <= 576  # u.obj: 288 quads -> 576 triangles, culled
+    ref = Path("/root/reference/config.txt")
+    if ref.exists():
+        sys_path = str(golden)
+        import sys
+
+        sys.path.insert(0, sys_path)
+        try:
+            import make_reference_scene as mk
+        finally:
+            sys.path.remove(sys_path)
+        vals, p2, t2 = mk.build(ref)
+        np.testing.assert_array_equal(vals, cfg)
+        np.testing.assert_array_equal(p2, parts)
+        np.testing.assert_array_equal(t2, tris)

@@ -120,34 +120,42 @@ def autotune(eng, trial_substeps: int = 8, warmup: int = 2, chunks=CHUNKS, launc
 
 class OnlineTuner:
     """Runs a caller's frames (``frame_substeps`` substeps each) and tunes
-    CHUNK on the way.  In every ``period`` frames (at least 4): frame
-    period-3 is timed at the current CHUNK; at frame period-2 CHUNK moves to
-    a neighbouring value in ``chunks`` (alternately the next larger and the
-    next smaller), and that frame runs untimed, taking the change's re-sort,
-    the new layout's first launches and, with graph replay, the capture of
-    the new frame graph; frame period-1 is timed at the neighbour, which is
-    kept when it was faster by more than ``margin``, otherwise CHUNK goes
-    back.  Nothing is rewound: every frame is a frame of the run.  Only the
-    timed frames synchronise with the device, and ``history`` records for
-    each trial whether either timed frame captured a graph (it should not:
-    the frames before them ran the same schedule)."""
+    CHUNK on the way.  Once per ``period`` frames (at least 4) a trial runs
+    over consecutive frames: one frame is timed at the current CHUNK; the
+    next moves CHUNK to a neighbouring value in ``chunks`` (alternately the
+    next larger and the next smaller) and runs untimed, taking the change's
+    re-sort, the new layout's first launches and, with graph replay, the
+    capture of the new frame graph; the frame after it is timed at the
+    neighbour, which is kept when it was faster by more than ``margin``,
+    otherwise CHUNK goes back.  A timed frame that captured a graph (the
+    engine's start state can alternate between calls, so a new setting may
+    need one capture per state) does not count: the next frame is timed in
+    its place, up to ``retries`` times.  Nothing is rewound: every frame is a
+    frame of the run, and only the timed frames synchronise with the device.
+    With no graph replay the trial occupies the last three frames of each
+    period."""
 
-    def __init__(self, eng, frame_substeps: int, period: int = 10, margin: float = 0.03, chunks=CHUNKS):
+    def __init__(self, eng, frame_substeps: int, period: int = 10, margin: float = 0.03, chunks=CHUNKS,
+                 retries: int = 3):
         if frame_substeps < 1:
             raise ValueError("frame_substeps must be at least 1")
         self.eng = eng
         self.k = int(frame_substeps)
         self.period = max(4, int(period))
         self.margin = float(margin)
+        self.retries = int(retries)
         self.chunks = tuple(sorted(set(int(c) for c in chunks)))
         self.frames = 0
         self.trials = 0
         self.switches = 0
-        # (frame, CHUNK tried, its ms, the current CHUNK's ms, kept, a timed frame captured a graph)
+        # (frame, CHUNK tried, its ms, the current CHUNK's ms, kept, timed frames retried for a capture)
         self.history = []
         self._up = True
+        self._idle = 0           # frames since the last trial ended
+        self._state = "idle"
+        self._tries = 0
+        self._retried = 0
         self._ref_ms = None
-        self._ref_cap = False
         self._cur = None
         self._cand = None
 
@@ -174,29 +182,44 @@ class OnlineTuner:
     def frame(self) -> None:
         """Runs one frame."""
         e = self.eng
-        phase = self.frames % self.period
         self.frames += 1
-        if phase == self.period - 3:
-            self._ref_ms, self._ref_cap = self._timed()
-            return
-        if phase == self.period - 2 and self._ref_ms is not None:
-            self._cur = e.get_option("CHUNK")
-            self._cand = self._neighbour(self._cur)
-            if self._cand is not None:
-                e.set_option("CHUNK", self._cand)
+        if self._state == "idle":
+            self._idle += 1
+            if self._idle >= self.period - 3:
+                self._state, self._tries = "ref", 0
             e.steps(self.k)
             return
-        if phase == self.period - 1 and self._cand is not None:
+        if self._state == "ref":
             ms, cap = self._timed()
-            self.trials += 1
-            kept = ms < self._ref_ms * (1.0 - self.margin)
-            if kept:
-                self.switches += 1
-            else:
-                e.set_option("CHUNK", self._cur)
-            self.history.append((self.frames - 1, self._cand, round(ms, 4), round(self._ref_ms, 4), kept,
-                                 bool(cap or self._ref_cap)))
-            self._ref_ms = None
-            self._cand = None
+            if cap and self._tries < self.retries:
+                self._tries += 1
+                self._retried += 1
+                return
+            self._ref_ms = ms
+            self._cur = e.get_option("CHUNK")
+            self._cand = self._neighbour(self._cur)
+            self._state = "switch" if self._cand is not None else "idle"
+            if self._cand is None:
+                self._idle = 0
             return
-        e.steps(self.k)
+        if self._state == "switch":
+            e.set_option("CHUNK", self._cand)
+            e.steps(self.k)
+            self._state, self._tries = "cand", 0
+            return
+        # "cand"
+        ms, cap = self._timed()
+        if cap and self._tries < self.retries:
+            self._tries += 1
+            self._retried += 1
+            return
+        self.trials += 1
+        kept = ms < self._ref_ms * (1.0 - self.margin)
+        if kept:
+            self.switches += 1
+        else:
+            e.set_option("CHUNK", self._cur)
+        self.history.append((self.frames - 1, self._cand, round(ms, 4), round(self._ref_ms, 4), kept,
+                             self._retried))
+        self._state, self._idle, self._retried = "idle", 0, 0
+        self._ref_ms = self._cand = None

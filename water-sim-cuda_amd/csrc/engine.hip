@@ -111,8 +111,10 @@ struct mpm_engine {
     int64_t last_claim_losers = 0;
     int64_t last_rounds = 0;
 
-    // Cached graph of one steps(k) call, valid for the start state it was
-    // captured from and the launch size it was captured with.
+    // Cached graphs of steps(k) calls, each valid for the start state it was
+    // captured from and the launch size it was captured with.  A few are kept
+    // because consecutive calls can start from alternating states (an odd
+    // number of re-sorts per call swaps the particle plane sets each call).
     struct GraphEntry {
         hipGraphExec_t exec = nullptr;
         int k = 0;
@@ -122,7 +124,10 @@ struct mpm_engine {
         // end state
         int64_t end_phase = 0;
         int end_vcur = 0, end_pcur = 0;
-    } graph;
+        int64_t used = 0;   // last use (for eviction)
+    };
+    static constexpr int GRAPH_CACHE = 4;
+    GraphEntry graphs[GRAPH_CACHE];
     int64_t graph_replays = 0;
     int64_t graph_captures = 0;
 
@@ -202,8 +207,10 @@ template <class T> void dev_free(T*& p) {
 }
 
 void drop_graph(mpm_engine* h) {
-    if (h->graph.exec) hipGraphExecDestroy(h->graph.exec);
-    h->graph = mpm_engine::GraphEntry{};
+    for (auto& g : h->graphs) {
+        if (g.exec) hipGraphExecDestroy(g.exec);
+        g = mpm_engine::GraphEntry{};
+    }
 }
 
 void free_particles(mpm_engine* h) {
@@ -473,13 +480,24 @@ int enqueue_steps(mpm_engine* h, int k, int launch) {
 // (schedule phase, buffer parities, k) and a launch size still fitting the
 // chunk count; otherwise captures a new one.
 int steps_graph(mpm_engine* h, int k) {
-    mpm_engine::GraphEntry& g = h->graph;
     const int64_t phase = h->substeps - h->sorted_at;
     const int want = transfer_launch_size(h);
+    // an entry for this start state (launch size aside), else the least
+    // recently used slot
+    mpm_engine::GraphEntry* slot = nullptr;
+    for (auto& c : h->graphs)
+        if (c.exec && c.k == k && c.phase == phase && c.vcur == h->vcur && c.pcur == h->pcur) slot = &c;
+    if (!slot) {
+        slot = &h->graphs[0];
+        for (auto& c : h->graphs)
+            if (!c.exec || (slot->exec && c.used < slot->used)) slot = &c;
+    }
+    mpm_engine::GraphEntry& g = *slot;
     const bool hit = g.exec && g.k == k && g.phase == phase && g.vcur == h->vcur && g.pcur == h->pcur &&
                      want <= g.launch && want * 2 >= g.launch;
     if (!hit) {
-        drop_graph(h);
+        if (g.exec) hipGraphExecDestroy(g.exec);
+        g = mpm_engine::GraphEntry{};
         mpm_engine::GraphEntry e;
         e.k = k;
         e.phase = phase;
@@ -516,6 +534,7 @@ int steps_graph(mpm_engine* h, int k) {
     }
     HIPCHK(hipGraphLaunch(g.exec, h->stream));
     ++h->graph_replays;
+    g.used = h->graph_replays;
     h->substeps += k;
     h->sorted_at = h->substeps - g.end_phase;
     h->vcur = g.end_vcur;

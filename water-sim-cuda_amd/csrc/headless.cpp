@@ -4,10 +4,14 @@
 //
 //   mpm_headless config.txt [--frames N] [--out DIR] [--every K] [--snapshots]
 //                [--checkpoint] [--resume FILE] [--resort R] [--graph]
+//                [--write-scene FILE]
+//   mpm_headless --scene FILE [same options]
 //
 // The scene comes from the config exactly as in the reference's Simulation
 // constructor (WATER meshes voxelised into particles, OBJECT meshes into the
-// collider SDF).  Each frame runs SUBSTEPS substeps (Simulation::run).  Every
+// collider SDF).  --write-scene saves that scene (config, the 26 x n buffer
+// the reference hands to Engine::init, the triangles it hands to initSDF) and
+// --scene runs a saved one without the config's mesh files.  Each frame runs SUBSTEPS substeps (Simulation::run).  Every
 // K frames one JSON line of statistics goes to stdout (and DIR/stats.jsonl):
 // frame, simulated time, particles, grid mass, height statistics, collider
 // penetration, mean |det F - 1|, wall time.  --snapshots writes the AoS
@@ -19,6 +23,8 @@
 //   .pos  "MPMP" u32 version=1 u32 n u32 frame f64 time, then n x 3 f32 (x, y, z)
 //   .bin  "MPMS" u32 version=1 u32 n u32 frame f64 time, then 26 x n f32
 //         (attribute-major, the reference's ParticleAttr layout, caller order)
+//   scene "MPMC" u32 version=1 u32 n u32 nt, then mpm_sim_config (14 x 4 B),
+//         26 x n f32 (as .bin) and nt x mpm_triangle (9 f32 each)
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -70,9 +76,46 @@ bool read_state(const std::string& path, std::vector<float>& state, uint32_t& n,
     return (bool)f;
 }
 
+struct SceneHeader {
+    char magic[4];
+    uint32_t version, n, nt;
+};
+
+bool write_scene(const std::string& path, const mpm_sim_config& cfg, const std::vector<float>& buf,
+                 const std::vector<mpm_triangle>& tris) {
+    std::ofstream f(path, std::ios::binary);
+    if (!f) return false;
+    SceneHeader h;
+    std::memcpy(h.magic, "MPMC", 4);
+    h.version = 1;
+    h.n = (uint32_t)(buf.size() / 26);
+    h.nt = (uint32_t)tris.size();
+    f.write(reinterpret_cast<const char*>(&h), sizeof(h));
+    f.write(reinterpret_cast<const char*>(&cfg), sizeof(cfg));
+    f.write(reinterpret_cast<const char*>(buf.data()), (std::streamsize)(buf.size() * sizeof(float)));
+    f.write(reinterpret_cast<const char*>(tris.data()), (std::streamsize)(tris.size() * sizeof(mpm_triangle)));
+    return (bool)f;
+}
+
+bool read_scene(const std::string& path, mpm_sim_config& cfg, std::vector<float>& buf,
+                std::vector<mpm_triangle>& tris) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    SceneHeader h;
+    f.read(reinterpret_cast<char*>(&h), sizeof(h));
+    if (!f || std::memcmp(h.magic, "MPMC", 4) != 0 || h.version != 1) return false;
+    f.read(reinterpret_cast<char*>(&cfg), sizeof(cfg));
+    buf.resize((size_t)26 * h.n);
+    tris.resize(h.nt);
+    f.read(reinterpret_cast<char*>(buf.data()), (std::streamsize)(buf.size() * sizeof(float)));
+    f.read(reinterpret_cast<char*>(tris.data()), (std::streamsize)(tris.size() * sizeof(mpm_triangle)));
+    return (bool)f && mpm_config_derive(&cfg) == MPM_OK;
+}
+
 int usage() {
     std::cerr << "usage: mpm_headless config.txt [--frames N] [--out DIR] [--every K] [--snapshots] [--checkpoint]\n"
-                 "                    [--resume FILE] [--resort R] [--graph]\n";
+                 "                    [--resume FILE] [--resort R] [--graph] [--write-scene FILE]\n"
+                 "       mpm_headless --scene FILE [same options]\n";
     return 2;
 }
 
@@ -80,10 +123,15 @@ int usage() {
 
 int main(int argc, char** argv) {
     if (argc < 2) return usage();
-    std::string config = argv[1], out, resume;
+    std::string config, out, resume, scene_in, scene_out;
     int frames = 10, every = 1, resort = 1;
     bool snapshots = false, checkpoint = false, graph = false;
-    for (int i = 2; i < argc; ++i) {
+    int first = 1;
+    if (std::string(argv[1]).rfind("--", 0) != 0) {
+        config = argv[1];
+        first = 2;
+    }
+    for (int i = first; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&]() -> const char* { return i + 1 < argc ? argv[++i] : nullptr; };
         const char* v = nullptr;
@@ -95,8 +143,11 @@ int main(int argc, char** argv) {
         else if (a == "--snapshots") snapshots = true;
         else if (a == "--checkpoint") checkpoint = true;
         else if (a == "--graph") graph = true;
+        else if (a == "--scene" && (v = next())) scene_in = v;
+        else if (a == "--write-scene" && (v = next())) scene_out = v;
         else return usage();
     }
+    if (config.empty() == scene_in.empty()) return usage();  // exactly one scene source
     if ((snapshots || checkpoint) && out.empty()) {
         std::cerr << "--snapshots/--checkpoint need --out DIR\n";
         return 2;
@@ -106,10 +157,26 @@ int main(int argc, char** argv) {
         mpm_sim_config cfg{};
         int64_t n = 0;
         int nt = 0;
-        mpm::check(mpm_scene_from_config(config.c_str(), &cfg, nullptr, 0, &n, nullptr, 0, &nt));
-        std::vector<float> buf((size_t)26 * n);
-        std::vector<mpm_triangle> tris((size_t)nt);
-        mpm::check(mpm_scene_from_config(config.c_str(), &cfg, buf.data(), n, &n, tris.data(), nt, &nt));
+        std::vector<float> buf;
+        std::vector<mpm_triangle> tris;
+        if (!scene_in.empty()) {
+            if (!read_scene(scene_in, cfg, buf, tris)) {
+                std::cerr << "cannot read scene " << scene_in << "\n";
+                return 1;
+            }
+            n = (int64_t)(buf.size() / 26);
+            nt = (int)tris.size();
+        } else {
+            mpm::check(mpm_scene_from_config(config.c_str(), &cfg, nullptr, 0, &n, nullptr, 0, &nt));
+            buf.resize((size_t)26 * n);
+            tris.resize((size_t)nt);
+            mpm::check(mpm_scene_from_config(config.c_str(), &cfg, buf.data(), n, &n, tris.data(), nt, &nt));
+        }
+        if (!scene_out.empty() && !write_scene(scene_out, cfg, buf, tris)) {
+            std::cerr << "cannot write scene " << scene_out << "\n";
+            return 1;
+        }
+        if (frames <= 0) return 0;
         uint32_t frame0 = 0;
         double t0 = 0.0;
         if (!resume.empty()) {

@@ -111,6 +111,55 @@ __device__ __forceinline__ void g2p_gather(Fetch fetch, const float wx[3], const
     for (int i = 0; i < 9; ++i) C[i] *= 4.0f;
 }
 
+// The same sums with the 27 node velocities delivered nine at a time, one
+// x slice (j, k = 0..2) per call of `slice(i, vx, vy, vz)`; used where some
+// nodes come from global memory, so that a slice's loads are all issued
+// before the first of them is used (one memory latency per slice, not one per
+// node).
+template <class Slice>
+__device__ __forceinline__ void g2p_gather_slices(Slice slice, const float wx[3], const float wy[3],
+                                                  const float wz[3], const float dx[3], const float dy[3],
+                                                  const float dz[3], float vel[3], float C[9]) {
+    float wxd[3], wyd[3], wzd[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        wxd[i] = wx[i] * dx[i];
+        wyd[i] = wy[i] * dy[i];
+        wzd[i] = wz[i] * dz[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float vx[9], vy[9], vz[9];
+        slice(i, vx, vy, vz);
+        float S[3] = {0.f, 0.f, 0.f}, Y[3] = {0.f, 0.f, 0.f}, T[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            float s[3] = {0.f, 0.f, 0.f}, t[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int n = 3 * j + k;
+                s[0] += wz[k] * vx[n]; s[1] += wz[k] * vy[n]; s[2] += wz[k] * vz[n];
+                t[0] += wzd[k] * vx[n]; t[1] += wzd[k] * vy[n]; t[2] += wzd[k] * vz[n];
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                S[a] += wy[j] * s[a];
+                Y[a] += wyd[j] * s[a];
+                T[a] += wy[j] * t[a];
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            vel[a] += wx[i] * S[a];
+            C[3 * a + 0] += wxd[i] * S[a];
+            C[3 * a + 1] += wx[i] * Y[a];
+            C[3 * a + 2] += wx[i] * T[a];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) C[i] *= 4.0f;
+}
+
 // getSDF at p from the .w lanes of the LDS tile when the sample's 2x2x2 cell
 // lies inside the tile and needs no clamping (then both read the same node
 // values); otherwise from the global SDF.
@@ -278,6 +327,8 @@ __device__ __forceinline__ void lds_add4(float4* slot, const float4 v) {
     atomicAdd(n + 3, v.w);
 }
 
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
+
 __device__ __forceinline__ bool node_in_tile(int tx, int ty, int tz) {
     return (unsigned)tx < (unsigned)TILE_DX && (unsigned)ty < (unsigned)TILE_DY && (unsigned)tz < (unsigned)TILE_DZ;
 }
@@ -299,6 +350,10 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
     // tile nodes) run on the same XCD and share its L2.
     const uint32_t nch = *a.nchunks;
     const int per = (int)((nch + 7u) >> 3);
+    // Diagnostic counts (particles outside their tile, lost claims), kept
+    // wave-uniform and added to the device counters once per wave: one global
+    // atomic per lane on a single address serialises at the memory side.
+    uint32_t n_outside = 0u, n_lost = 0u;
     const int xcd = blockIdx.x & 7, nl = gridDim.x >> 3;
     for (int t = blockIdx.x >> 3; t < per; t += nl) {
     const int ci = __builtin_amdgcn_readfirstlane(xcd * per + t);
@@ -410,18 +465,34 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
                         wx, wy, wz, dx, dy, dz, vel, C);
                 } else {
                     // stencil partly outside the tile: nodes inside it from the
-                    // tile (the same values), the others from global memory
-                    g2p_gather(
-                        [&](int i, int jj, int k) {
-                            const int x = bx + i, y = by + jj, z = bz + k;
-                            const int tx = x - d.ox, ty = y - d.oy, tz = z - d.oz;
-                            if (node_in_tile(tx, ty, tz)) {
-                                const float4 v = vt[tx + TILE_PY * ty + TILE_PZ * tz];
-                                return make_float3(v.x, v.y, v.z);
+                    // tile (the same values), the others from global memory.
+                    // A slice's nine nodes are loaded unconditionally at
+                    // coordinates clamped into the grid, so the 27 loads are in
+                    // flight together; nodes in the tile or outside the grid
+                    // are then taken from the tile or set to zero.
+                    g2p_gather_slices(
+                        [&](int i, float vx[9], float vy[9], float vz[9]) {
+                            const int x = bx + i;
+                            const int xc = min(max(x, 0), c.sx - 1);
+#pragma unroll
+                            for (int n = 0; n < 9; ++n) {
+                                const int y = by + n / 3, z = bz + n % 3;
+                                const int g = gidx(c, xc, min(max(y, 0), c.sy - 1), min(max(z, 0), c.sz - 1));
+                                vx[n] = a.vel[0][g];
+                                vy[n] = a.vel[1][g];
+                                vz[n] = a.vel[2][g];
                             }
-                            if (!inb(c, x, y, z)) return make_float3(0.f, 0.f, 0.f);
-                            const int g = gidx(c, x, y, z);
-                            return make_float3(a.vel[0][g], a.vel[1][g], a.vel[2][g]);
+#pragma unroll
+                            for (int n = 0; n < 9; ++n) {
+                                const int y = by + n / 3, z = bz + n % 3;
+                                const int tx = x - d.ox, ty = y - d.oy, tz = z - d.oz;
+                                if (node_in_tile(tx, ty, tz)) {
+                                    const float4 v = vt[tx + TILE_PY * ty + TILE_PZ * tz];
+                                    vx[n] = v.x; vy[n] = v.y; vz[n] = v.z;
+                                } else if (!inb(c, x, y, z)) {
+                                    vx[n] = 0.f; vy[n] = 0.f; vz[n] = 0.f;
+                                }
+                            }
                         },
                         wx, wy, wz, dx, dy, dz, vel, C);
                 }
@@ -457,7 +528,6 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
                     // the nodes inside it (before any lane's read-modify-write
                     // chain below, in program order), global atomics for the
                     // others
-                    atomicAdd(a.overflow, 1u);
                     p2g_scatter(c, p, pq, [&](int i, int jj, int k, const P2GNode& nd) {
                         const int x = bx + i, y = by + jj, z = bz + k;
                         const int tx = x - d.ox, ty = y - d.oy, tz = z - d.oz;
@@ -497,14 +567,15 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
                     asm volatile("" ::: "memory");  // keep the 27 updates in program order
                 });
             };
+            n_outside += (uint32_t)__popcll(__ballot(valid && !in_tile));
             if (in_tile) claim[li] = (uint8_t)lane;
             asm volatile("" ::: "memory");
             const bool mine = in_tile && claim[li] == (uint8_t)lane;
             asm volatile("" ::: "memory");
             if (mine) arena_chain(p, pq);
             const bool lost = in_tile && !mine;
+            n_lost += (uint32_t)__popcll(__ballot(lost));
             if (lost) {
-                atomicAdd(a.overflow + 1, 1u);
                 Particle p2 = p;
                 P2GPrep q2 = pq;
                 asm volatile("" : "+v"(p2.vx), "+v"(p2.vy), "+v"(p2.vz));
@@ -542,6 +613,12 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
         }
         __syncthreads();  // the next chunk reuses the tile and the arena
     }
+    }
+    if constexpr (DO_P2G) {
+        if (lane_id() == 0) {
+            if (n_outside) atomicAdd(a.overflow, n_outside);
+            if (n_lost) atomicAdd(a.overflow + 1, n_lost);
+        }
     }
 }
 

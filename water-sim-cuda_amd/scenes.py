@@ -98,6 +98,29 @@ def sphere_sdf(cfg, centre, radius: float) -> np.ndarray:
     return d.astype(np.float32).reshape(-1)
 
 
+def sample_sdf(cfg, sdf: np.ndarray, pos: np.ndarray) -> np.ndarray:
+    """Trilinear node-SDF values at (3, n) positions, cells clamped into the
+    grid as getSDF does (common.cuh:136-167); float64."""
+    sx, sy, sz = int(cfg.SIZE_X), int(cfg.SIZE_Y), int(cfg.SIZE_Z)
+    p = np.asarray(pos, dtype=np.float64)
+    i0 = np.floor(p).astype(np.int64)
+    t = p - i0
+    dims = np.array([sx, sy, sz])[:, None]
+    lo = np.clip(i0, 0, dims - 1)
+    hi = np.clip(i0 + 1, 0, dims - 1)
+    s = np.asarray(sdf, dtype=np.float64).reshape(-1)
+    out = np.zeros(p.shape[1])
+    for cz in (0, 1):
+        for cy in (0, 1):
+            for cx in (0, 1):
+                x = hi[0] if cx else lo[0]
+                y = hi[1] if cy else lo[1]
+                z = hi[2] if cz else lo[2]
+                w = ((t[0] if cx else 1 - t[0]) * (t[1] if cy else 1 - t[1]) * (t[2] if cz else 1 - t[2]))
+                out += w * s[x + sx * (y + sy * z)]
+    return out
+
+
 def batch_scenes(count: int = 64, seed: int = 4, n_min: int = 5000, n_max: int = 400000,
                  edges=(32, 48, 64, 96), physical: bool = True) -> dict:
     """Independent scenes for the batched engine.
@@ -108,6 +131,8 @@ def batch_scenes(count: int = 64, seed: int = 4, n_min: int = 5000, n_max: int =
     per cell); a water sphere or cube of that volume at 8 ppc placed at random
     in the upper half; a random collider, either a box floor (y below a random
     height) or a solid sphere below the water, as an analytic node SDF.
+    Particles that would start inside the collider (node SDF interpolated at
+    their position below zero) are left out, so no water starts in a solid.
     DT 1e-3, 20 substeps per frame, PADDING 3.
 
     Returns ``cfgs``, ``counts``, ``particles`` (26 x sum(counts), scene after
@@ -144,6 +169,7 @@ def batch_scenes(count: int = 64, seed: int = 4, n_min: int = 5000, n_max: int =
         else:
             rc = float(rng.uniform(e / 8, e / 5))
             sdf = sphere_sdf(cfg, (rng.uniform(e / 3, 2 * e / 3), 3.0 + 0.5 * rc, rng.uniform(e / 3, 2 * e / 3)), rc)
+        p = np.ascontiguousarray(p[:, sample_sdf(cfg, sdf, p[0:3]) >= 0.0])
         cfgs.append(cfg)
         parts.append(p)
         sdfs.append(sdf)
