@@ -258,6 +258,35 @@ int mpm_engine_debug_g2p(mpm_engine* h, const float* grid4_in);
 int mpm_host_alloc(size_t bytes, void** out);
 int mpm_host_free(void* p);
 
+/* ---- device memory for callers that stage their own buffers (no reference
+ * counterpart; the reference hands out device pointers through
+ * Engine::getBuffer/getParticles, engine.h:22, :32) ----
+ * Plain device allocations on the current HIP device and synchronous copies,
+ * for the entry points below that take device pointers. */
+int mpm_device_alloc(size_t bytes, void** out);
+int mpm_device_free(void* p);
+int mpm_copy_to_device(void* dst, const void* src, size_t bytes);
+int mpm_copy_to_host(void* dst, const void* src, size_t bytes);
+
+/* ---- key sort that allocates nothing ----
+ * No counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It computes what mpm_engine_get_sort
+ * returns -- the Morton keys of setKeys (kernels.cu:361-371) and their stable
+ * ascending sort with the particle indices (Engine::sortParticles,
+ * engine.cu:92-102, which allocates its keys every substep, engine.cu:95) --
+ * for n particles whose positions are three device arrays px, py, pz, into
+ * the device arrays keys_out (sorted keys) and perm_out (indices in sorted
+ * order).  Work is enqueued on `stream` (a hipStream_t; NULL for the null
+ * stream) and the call returns without waiting.  Its scratch is a fixed
+ * device array of about 34 MB per GPU, reserved by the library's code object
+ * for the life of the process: n above MPM_SORT_NOALLOC_MAX returns
+ * MPM_ELIMIT.  Calls share that array, so calls that overlap on one GPU
+ * (other streams or host threads) race; serialise them. */
+#define MPM_SORT_NOALLOC_MAX (1 << 21)
+int mpm_sort_keys_noalloc(const float* px, const float* py, const float* pz, int n, int32_t* keys_out,
+                          int32_t* perm_out, void* stream);
+
 /* ---- measurement aid (no reference counterpart) ---- */
 
 /* Copies `bytes` (rounded down to 16) between two fresh device buffers

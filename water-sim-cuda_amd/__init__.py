@@ -32,6 +32,8 @@ ATTR_NAMES = (
 )
 
 MPM_OK = 0
+MPM_ELIMIT = -6
+SORT_NOALLOC_MAX = 1 << 21  # mpm_sort_keys_noalloc's fixed capacity (MPM_SORT_NOALLOC_MAX)
 
 
 class MPMError(RuntimeError):
@@ -164,6 +166,12 @@ def load_library() -> ctypes.CDLL:
         "mpm_probe_copy": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, c_dbl_p]),
         "mpm_host_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
         "mpm_host_free": (ctypes.c_int, [ctypes.c_void_p]),
+        "mpm_device_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+        "mpm_device_free": (ctypes.c_int, [ctypes.c_void_p]),
+        "mpm_copy_to_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+        "mpm_copy_to_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+        "mpm_sort_keys_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
         "mpm_obj_triangles": (ctypes.c_int, [ctypes.c_char_p, c_float_p, ctypes.c_int, c_int_p]),
         "mpm_voxelize_obj": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_float, c_float_p, ctypes.c_int64, c_i64_p]),
         "mpm_obj_water": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_float, ctypes.c_float, c_float_p, ctypes.c_int,

@@ -36,7 +36,8 @@ HIP_FLAGS = [
 # -ffp-contract=off: the scene front-end restates FP32 arithmetic step by step
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 
-HIP_SOURCES = ["kernels_transfer.hip", "kernels_grid.hip", "kernels_sort.hip", "engine.hip", "corrector.hip"]
+HIP_SOURCES = ["kernels_transfer.hip", "kernels_grid.hip", "kernels_sort.hip", "engine.hip", "corrector.hip",
+               "noalloc.hip"]
 CXX_SOURCES = ["config.cpp", "scene.cpp", "voxel.cpp"]
 
 

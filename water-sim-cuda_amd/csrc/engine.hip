@@ -1354,6 +1354,36 @@ int mpm_host_free(void* p) {
     return MPM_OK;
 }
 
+int mpm_device_alloc(size_t bytes, void** out) {
+    if (!out) return fail(MPM_EINVAL, "null output");
+    *out = nullptr;
+    if (bytes == 0) return MPM_OK;
+    if (hipMalloc(out, bytes) != hipSuccess) {
+        *out = nullptr;
+        return fail(MPM_ENOMEM, "hipMalloc failed");
+    }
+    return MPM_OK;
+}
+
+int mpm_device_free(void* p) {
+    if (p && hipFree(p) != hipSuccess) return fail(MPM_EHIP, "hipFree failed");
+    return MPM_OK;
+}
+
+int mpm_copy_to_device(void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return MPM_OK;
+    if (!dst || !src) return fail(MPM_EINVAL, "null argument");
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return MPM_OK;
+}
+
+int mpm_copy_to_host(void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return MPM_OK;
+    if (!dst || !src) return fail(MPM_EINVAL, "null argument");
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return MPM_OK;
+}
+
 int mpm_probe_copy(size_t bytes, int iters, double* ms_per_copy) {
     if (bytes < 16 || iters < 1 || !ms_per_copy) return fail(MPM_EINVAL, "bad copy probe arguments");
     const size_t n = bytes / 16;
