@@ -121,3 +121,45 @@ def test_layout_many_chunks_per_workgroup(gpu):
     assert err < 3e-4
     small.close()
     whole.close()
+
+
+def test_round_count_hysteresis(gpu):
+    """A block keeps the round count of its previous layout while that is at
+    most one above the fewest it needs (a different count re-deals every
+    particle of the chunk, scattering the next transfer's reads): one 8x4x4
+    block of base cells with 8 particles per cell plus one (1025: 17 rounds),
+    then the 1024 of a full block (16 needed, 17 kept), then 7 per cell (14
+    needed, 17 dropped).  A fresh engine lays out the fewest."""
+    cfg = gpu.make_config(SIZE_X=32, SIZE_Y=32, SIZE_Z=32)
+    rng = np.random.default_rng(3)
+
+    def block_state(per_cell, extra):
+        cells = np.stack(np.meshgrid(np.arange(8, 16), np.arange(8, 12), np.arange(8, 12), indexing="ij"),
+                         0).reshape(3, -1)
+        pos = np.repeat(cells, per_cell, axis=1).astype(np.float64)
+        if extra:
+            pos = np.concatenate([pos, np.array([[8.0], [8.0], [8.0]])], axis=1)
+        pos = pos + 1.0 + rng.uniform(-0.4, 0.4, size=pos.shape)  # base cell trunc(x - 0.5) = the cell
+        n = pos.shape[1]
+        st = np.zeros((26, n), dtype=np.float32)
+        st[0:3] = pos
+        st[[15, 19, 23]] = 1.0
+        st[24] = st[25] = 1.0
+        return st
+
+    e = gpu.Engine(cfg, CHUNK=2048)
+    e.load_state(block_state(8, True))
+    r1025 = e.counters()["rounds"]
+    e.load_state(block_state(8, False))
+    r1024 = e.counters()["rounds"]
+    e.load_state(block_state(7, False))
+    r896 = e.counters()["rounds"]
+    e.steps(2)
+    assert np.isfinite(e.getParticles()).all()
+    e.close()
+    fresh = gpu.Engine(cfg, CHUNK=2048)
+    fresh.load_state(block_state(8, False))
+    r_fresh = fresh.counters()["rounds"]
+    fresh.close()
+    say(f"rounds: 1025 -> {r1025}, then 1024 -> {r1024}, then 896 -> {r896}; fresh 1024 -> {r_fresh}")
+    assert (r1025, r1024, r896, r_fresh) == (17, 17, 14, 16)

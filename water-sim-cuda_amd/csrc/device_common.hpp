@@ -85,7 +85,7 @@ struct ChunkDesc {
     int ox, oy, oz;  // node origin of the chunk's LDS tile
     int S;           // rounds
     int scene;       // scene of the chunk (index into the DevCfg table)
-    int pad1;
+    int block;       // its chunk block when the block is one chunk, else -1
 };
 
 // Chunk block: 8 x 4 x 4 base cells, i.e. two x-adjacent 4^3 flag blocks

@@ -592,8 +592,9 @@ int build_block_table(mpm_engine* h) {
     int rc;
     if ((rc = dev_alloc(&b.rank, (size_t)h->nfb)) || (rc = dev_alloc(&b.block, (size_t)nb)) ||
         (rc = dev_alloc(&b.count, (size_t)nb)) || (rc = dev_alloc(&b.scan, (size_t)nb)) ||
-        (rc = dev_alloc(&b.partial, scan_partial_elems((size_t)nb) + 16)))
+        (rc = dev_alloc(&b.partial, scan_partial_elems((size_t)nb) + 16)) || (rc = dev_alloc(&b.prev_rounds, (size_t)nb)))
         return rc;
+    HIPCHK(hipMemset(b.prev_rounds, 0, (size_t)nb * sizeof(uint16_t)));
     HIPCHK(hipMemcpy(b.rank, rank.data(), rank.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b.block, block.data(), block.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(b.count, 0, (size_t)nb * sizeof(uint32_t)));
@@ -837,6 +838,7 @@ int mpm_engine_destroy(mpm_engine* h) {
     dev_free(h->blocks.count);
     dev_free(h->blocks.scan);
     dev_free(h->blocks.partial);
+    dev_free(h->blocks.prev_rounds);
     dev_free(h->sc.counters);
     h->overflow = nullptr;
     if (h->pinned) hipHostFree(h->pinned);

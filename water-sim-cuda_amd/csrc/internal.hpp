@@ -110,6 +110,11 @@ struct BlockTable {
     uint32_t* count = nullptr;    // nb: bucket sizes (all zero between re-sorts)
     uint64_t* scan = nullptr;     // nb: exclusive prefix of count | chunks << 32
     uint64_t* partial = nullptr;  // scan partials
+    // nb: the round count the block's chunk was last laid out with (0 before
+    // the first layout); a re-sort keeps it while it is within one of the
+    // fewest rounds the block needs, so that a count moving back and forth
+    // across a multiple of 64 does not re-deal every particle of the chunk
+    uint16_t* prev_rounds = nullptr;
     int nb = 0;
 };
 

@@ -411,7 +411,7 @@ __global__ __launch_bounds__(256) void k_bucket_chunks(uint32_t* __restrict__ bc
         d.oz = 4 * bz;
         d.S = 0;  // set by the reorder
         d.scene = scene;
-        d.pad1 = 0;
+        d.block = pieces == 1 ? b : -1;
         chunks[c] = d;
     }
 }
@@ -458,7 +458,8 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
                                                         const int* __restrict__ src_orig, float* __restrict__ dst,
                                                         size_t dst_stride, int* __restrict__ dst_orig,
                                                         uint32_t* __restrict__ rounds_total,
-                                                        uint32_t* __restrict__ map, uint32_t mask) {
+                                                        uint32_t* __restrict__ map, uint32_t mask,
+                                                        uint16_t* __restrict__ prev_rounds) {
     static_assert(BLOCK_CELLS_N == 128 && RO_THREADS >= 128, "one wave scans the cell counts, two per lane");
     extern __shared__ uint32_t ro_lds[];
     uint32_t* cell_rank = ro_lds;          // [cap] cell | atomic rank << BLOCK_BITS per chunk position;
@@ -582,7 +583,18 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
         load_orig();  // the next chunk's slots have arrived by now
         __syncthreads();
         const uint32_t n = (uint32_t)count;
-        const uint32_t S = max(cmax, (n + 63u) / 64u);
+        // The fewest rounds (a round holds distinct cells, at most 64), or the
+        // block's previous round count when that is at most one more: a
+        // different S re-deals every list entry (entry q goes to round q % S),
+        // so the next transfer would read the chunk's particles in scattered
+        // order, while an unchanged S moves them by whole rounds.
+        const uint32_t s_min = max(cmax, (n + 63u) / 64u);
+        const int blk = __builtin_amdgcn_readfirstlane(ch[ci].block);
+        uint32_t S = s_min;
+        if (blk >= 0) {
+            const uint32_t prev = prev_rounds[blk];
+            if (prev >= s_min && prev <= s_min + 1u) S = prev;
+        }
         const uint32_t nl = (n + S - 1u) / S;     // lanes of the first rounds (<= 64 as S >= n / 64)
         const uint32_t full = n - (nl - 1u) * S;  // rounds with nl lanes, 1 .. S
         // one wave per round: lane L holds list entry L * S + r
@@ -611,6 +623,7 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
         if (tid == 0) {
             ch[ci].S = (int)S;
             rsum += S;
+            if (blk >= 0) prev_rounds[blk] = (uint16_t)min(S, 65535u);
         }
         __syncthreads();
 #pragma unroll
@@ -722,11 +735,11 @@ void launch_resort(const ResortArgs& a, hipStream_t s) {
     if (a.mask)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_reorder<true>), dim3((unsigned)grid), dim3(RO_THREADS), lds, s, a.chunks,
                            a.counters, a.chunk_cap, a.keys[1], a.vals[1], a.src, a.src_stride, a.src_orig, a.dst,
-                           a.dst_stride, a.dst_orig, a.counters + 4, a.map, a.mask);
+                           a.dst_stride, a.dst_orig, a.counters + 4, a.map, a.mask, bt.prev_rounds);
     else
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_reorder<false>), dim3((unsigned)grid), dim3(RO_THREADS), lds, s, a.chunks,
                            a.counters, a.chunk_cap, a.keys[1], a.vals[1], a.src, a.src_stride, a.src_orig, a.dst,
-                           a.dst_stride, a.dst_orig, a.counters + 4, a.map, 0u);
+                           a.dst_stride, a.dst_orig, a.counters + 4, a.map, 0u, bt.prev_rounds);
 }
 
 }  // namespace mpm
