@@ -19,9 +19,11 @@ boundary.
 * `cpu_baseline` times the oracle's OpenMP step on the host threads this
   process may use, on a bounded sample (a few substeps of the same scene).
 
-`--scene batch64` measures the batched 64-scene workload instead; `--graph`
-replays each frame as a HIP graph; `--autotune` tunes CHUNK and LAUNCH_PCT
-first; `--tune-online F` keeps tuning CHUNK every F frames.
+`--scene batch64` measures the batched 64-scene workload instead, with its
+block and tile sizes chosen by the autotuner as that workload specifies
+(trials on the warmed-up state, then a CHUNK trial every 10 frames of the
+run; `--no-tune` keeps the hand-set CHUNK); `--graph` replays each frame as a
+HIP graph; `--autotune` and `--tune-online F` tune any scene.
 """
 from __future__ import annotations
 
@@ -214,10 +216,13 @@ def main() -> int:
     ap.add_argument("--no-stage-table", action="store_true", help="skip the per-stage timing pass")
     ap.add_argument("--graph", action="store_true", help="replay each frame's steps() call as a HIP graph")
     ap.add_argument("--frame", type=int, default=0, help="substeps per steps() call [the scene's SUBSTEPS]")
-    ap.add_argument("--autotune", action="store_true", help="pick CHUNK and LAUNCH_PCT by timing trials first")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick CHUNK and LAUNCH_PCT by timing trials first [on for batch scenes]")
+    ap.add_argument("--no-tune", action="store_true", help="no autotuning of batch scenes (their hand-set CHUNK)")
     ap.add_argument("--tune-cache", default=None, help="JSON cache file for --autotune")
-    ap.add_argument("--tune-online", type=int, default=0, metavar="FRAMES",
-                    help="keep tuning CHUNK in the timed run: one trial every FRAMES frames (0: off)")
+    ap.add_argument("--tune-online", type=int, default=None, metavar="FRAMES",
+                    help="keep tuning CHUNK in the timed run: one trial every FRAMES frames (0: off) "
+                         "[10 for batch scenes, else 0]")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "g2p2g_channel256_traffic.txt"),
                     help="rocprofv3-derived HBM bytes per launch of the fused kernel (key: value text)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-corrector evaluation")
@@ -253,6 +258,14 @@ def main() -> int:
         n = sc["particles"].shape[1]
         grid_desc = [cfg.SIZE_X, cfg.SIZE_Y, cfg.SIZE_Z]
     resort = sc["resort_every"] if args.resort_every is None else args.resort_every
+    # the batched workload is defined with block and tile sizes chosen by the
+    # autotuner: tuned first on the warmed-up state, then online during the run
+    if batch and not args.no_tune:
+        args.autotune = True
+        if args.tune_online is None:
+            args.tune_online = 10
+    if args.tune_online is None or args.no_tune:
+        args.tune_online = 0
     log(f"scene {sc['name']}: n={n} grid={grid_desc} (host gen {time.time() - t0:.1f}s)")
 
     # The reference's frame loop: SUBSTEPS substeps per call (Simulation.cpp:42-54).

@@ -136,9 +136,10 @@ def batch_scenes(count: int = 64, seed: int = 4, n_min: int = 5000, n_max: int =
     DT 1e-3, 20 substeps per frame, PADDING 3.
 
     Returns ``cfgs``, ``counts``, ``particles`` (26 x sum(counts), scene after
-    scene), ``sdfs`` (one array per scene), ``resort_every`` and ``chunk``
-    (the fastest measured settings: the scenes are small and splash, so
-    smaller work items balance better) and ``description``.
+    scene), ``sdfs`` (one array per scene), ``resort_every`` (the fastest
+    measured period with the autotuner), ``chunk`` (the starting CHUNK before
+    tuning: the scenes are small and splash, so smaller work items balance
+    better late in a run) and ``description``.
     """
     m = _pkg()
     rng = np.random.default_rng(seed)
@@ -175,7 +176,7 @@ def batch_scenes(count: int = 64, seed: int = 4, n_min: int = 5000, n_max: int =
         sdfs.append(sdf)
         counts.append(p.shape[1])
     return dict(name=f"batch{count}", cfgs=cfgs, counts=np.array(counts, dtype=np.int32),
-                particles=np.concatenate(parts, axis=1), sdfs=sdfs, resort_every=2, chunk=256, frames=100,
+                particles=np.concatenate(parts, axis=1), sdfs=sdfs, resort_every=4, chunk=256, frames=100,
                 description=f"{count} scenes, edges {edges}, {n_min}-{n_max} particles log-uniform (capped at "
                             f"0.3*8*edge^3), sphere/cube water, floor or sphere collider, DT 1e-3")
 
