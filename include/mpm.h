@@ -372,6 +372,12 @@ int mpm_corrector_set_norm(mpm_corrector* c, const float* mean, const float* inv
 /* Limit on |network output| (before out_scale) in the forward pass and in
  * the engine [no limit]: a guard against extrapolation. */
 int mpm_corrector_set_clamp(mpm_corrector* c, float clamp);
+/* Dead zone of the forward pass and the engine [0]: a node whose network
+ * output has a magnitude (before out_scale) below `dead` gets no correction.
+ * The network's BF16 activations resolve about 1/256 of its output scale, so
+ * outputs that small are rounding, and applying them at every node of a
+ * resting fluid only adds noise. */
+int mpm_corrector_set_deadzone(mpm_corrector* c, float dead);
 /* Y[n][3] = corrections of the n feature rows X[n][72] (host arrays). */
 int mpm_corrector_forward(mpm_corrector* c, const float* X, int n, float* Y);
 /* Training set on the device: features X[n][72], target corrections T[n][3]. */

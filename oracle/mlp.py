@@ -63,8 +63,9 @@ def _q(a, on: bool):
     return bf16(a).astype(np.float64) if on else np.asarray(a, dtype=np.float64)
 
 
-def forward(X, params, mean, inv_std, out_scale=1.0, emulate_bf16=True, cache=False):
-    """Corrections (n, 3) of feature rows X (n, 66 or 72)."""
+def forward(X, params, mean, inv_std, out_scale=1.0, emulate_bf16=True, cache=False, dead=0.0):
+    """Corrections (n, 3) of feature rows X (n, 66 or 72); rows whose output
+    magnitude (before out_scale) is below `dead` give zero."""
     X = np.asarray(X, dtype=np.float32)[:, :NF]
     W = unpack(np.asarray(params, dtype=np.float32))
     xn = ((X - np.asarray(mean, np.float32)[None, :]) * np.asarray(inv_std, np.float32)[None, :]).astype(np.float32)
@@ -79,6 +80,8 @@ def forward(X, params, mean, inv_std, out_scale=1.0, emulate_bf16=True, cache=Fa
     y = h3 @ _q(W["W4"], q) + W["b4"].astype(np.float64)
     if cache:
         return y, (h0, h1, h2, h3)
+    if dead > 0.0:
+        y = np.where((np.sqrt((y * y).sum(1)) < dead)[:, None], 0.0, y)
     return y * out_scale
 
 

@@ -111,3 +111,19 @@ def test_node_features_hand_grid():
     assert f[64] == pytest.approx(min(2, 1, 2, 1, 2, 1) / 8.0)
     assert f[65] == 0.5
     assert not f[66:].any()
+
+
+def test_dead_zone_drops_small_outputs():
+    """forward(dead=d) zeroes exactly the rows whose output norm (before
+    out_scale) is below d and leaves the others as they are."""
+    rng = np.random.default_rng(3)
+    X = rng.normal(size=(500, 72)).astype(np.float32)
+    p = mlp.init_params(4)
+    y = mlp.forward(X, p, np.zeros(66), np.ones(66), 1.0, emulate_bf16=False)
+    norm = np.sqrt((y * y).sum(1))
+    d = float(np.median(norm))
+    z = mlp.forward(X, p, np.zeros(66), np.ones(66), 3.0, emulate_bf16=False, dead=d)
+    small = norm < d
+    assert small.any() and (~small).any()
+    assert not z[small].any()
+    np.testing.assert_allclose(z[~small], 3.0 * y[~small], rtol=1e-12)

@@ -151,6 +151,7 @@ def load_library() -> ctypes.CDLL:
         "mpm_corrector_get_params": (ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int]),
         "mpm_corrector_set_norm": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, ctypes.c_float]),
         "mpm_corrector_set_clamp": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
+        "mpm_corrector_set_deadzone": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
         "mpm_corrector_forward": (ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_float_p]),
         "mpm_corrector_load_dataset": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, ctypes.c_int]),
         "mpm_corrector_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_float_p, c_float_p]),
@@ -647,6 +648,10 @@ class Corrector:
 
     def set_clamp(self, clamp: float) -> None:
         _check(self._lib.mpm_corrector_set_clamp(self._h, float(clamp)))
+
+    def set_deadzone(self, dead: float) -> None:
+        """Outputs (before out_scale) of magnitude below `dead` give no correction."""
+        _check(self._lib.mpm_corrector_set_deadzone(self._h, float(dead)))
 
     @staticmethod
     def _rows(X) -> np.ndarray:

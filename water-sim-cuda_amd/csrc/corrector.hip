@@ -182,7 +182,18 @@ struct Norm {
     const float* inv_std;  // KIN (0 for padding)
     float out_scale;       // correction = network output * out_scale
     float clamp;           // |network output| limit of the forward pass
+    float dead;            // outputs of smaller magnitude give no correction
 };
+
+// One node's correction from the network's three outputs: zero when their
+// magnitude is below the dead zone, else each limited to the clamp and scaled.
+__device__ __forceinline__ void store_correction(float* __restrict__ Y, int row, float o0, float o1, float o2,
+                                                 const Norm& nm) {
+    const bool live = o0 * o0 + o1 * o1 + o2 * o2 >= nm.dead * nm.dead;
+    Y[(size_t)row * 3 + 0] = live ? fminf(fmaxf(o0, -nm.clamp), nm.clamp) * nm.out_scale : 0.0f;
+    Y[(size_t)row * 3 + 1] = live ? fminf(fmaxf(o1, -nm.clamp), nm.clamp) * nm.out_scale : 0.0f;
+    Y[(size_t)row * 3 + 2] = live ? fminf(fmaxf(o2, -nm.clamp), nm.clamp) * nm.out_scale : 0.0f;
+}
 
 // Forward over rows [0, *count) (or n when count is null): Y[row][3] =
 // out_scale * clamp(MLP(X[row])).  Four waves per workgroup share the forward
@@ -209,9 +220,7 @@ __global__ __launch_bounds__(256) void k_forward(const float* __restrict__ X, in
         if (valid && hi == 0) {
             // guard: outputs beyond the clamp (in target-RMS units, far
             // outside anything the training set holds) are limited
-            Y[(size_t)row * 3 + 0] = fminf(fmaxf(a4[0], -nm.clamp), nm.clamp) * nm.out_scale;
-            Y[(size_t)row * 3 + 1] = fminf(fmaxf(a4[1], -nm.clamp), nm.clamp) * nm.out_scale;
-            Y[(size_t)row * 3 + 2] = fminf(fmaxf(a4[2], -nm.clamp), nm.clamp) * nm.out_scale;
+            store_correction(Y, row, a4[0], a4[1], a4[2], nm);
         }
     }
 }
@@ -585,11 +594,7 @@ __global__ __launch_bounds__(256) void k_correct(const DevCfg* __restrict__ cfgs
             asm volatile("" ::: "memory");  // weight fragments are re-read from LDS per tile
             mlp_fwd(wfr, P, xb, lane, h1, h2, h3, a4);
             const int row = base + tt * 32 + (lane & 31);
-            if (row < rows && hi == 0) {
-                Y[(size_t)row * 3 + 0] = fminf(fmaxf(a4[0], -nm.clamp), nm.clamp) * nm.out_scale;
-                Y[(size_t)row * 3 + 1] = fminf(fmaxf(a4[1], -nm.clamp), nm.clamp) * nm.out_scale;
-                Y[(size_t)row * 3 + 2] = fminf(fmaxf(a4[2], -nm.clamp), nm.clamp) * nm.out_scale;
-            }
+            if (row < rows && hi == 0) store_correction(Y, row, a4[0], a4[1], a4[2], nm);
         }
         asm volatile("" ::: "memory");  // the next nodes overwrite xs
     }
@@ -720,14 +725,14 @@ void launch_corr_features(const DevCfg* cfgs, GridBuf V, const float* sdf, const
 }
 
 void launch_corr_forward(const CorrState& cs, const float* X, int cap, const uint32_t* count, float* Y, hipStream_t s) {
-    Norm nm{cs.mean, cs.inv_std, cs.out_scale, cs.out_clamp};
+    Norm nm{cs.mean, cs.inv_std, cs.out_scale, cs.out_clamp, cs.out_dead};
     hipLaunchKernelGGL(k_forward, dim3(grid_blocks(cap, 128, 1024)), dim3(256), 0, s, X, cap, count, nm,
                        (const bf16x4*)cs.frags, cs.params, Y);
 }
 
 void launch_corr_correct(const DevCfg* cfgs, GridBuf V, const float* sdf, const int2* list, const uint32_t* count,
                          int cap, const CorrState& cs, float* Y, hipStream_t s) {
-    Norm nm{cs.mean, cs.inv_std, cs.out_scale, cs.out_clamp};
+    Norm nm{cs.mean, cs.inv_std, cs.out_scale, cs.out_clamp, cs.out_dead};
     hipLaunchKernelGGL(k_correct, dim3(grid_blocks(cap, 128, 1024)), dim3(256), 0, s, cfgs, V, sdf, list, count, cap,
                        nm, (const bf16x4*)cs.frags, cs.params, Y);
 }
@@ -817,6 +822,7 @@ int mpm_corrector_create(mpm_corrector** out) {
     for (int i = 0; i < NF; ++i) inv[i] = 1.0f;
     c->st.out_scale = 1.0f;
     c->st.out_clamp = 1e30f;
+    c->st.out_dead = 0.0f;
     hipError_t e = hipMemcpy(c->st.params, zero.data(), sizeof(float) * CORR_PARAMS, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->grad, zero.data(), sizeof(float) * CORR_PARAMS, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->adam_m, zero.data(), sizeof(float) * CORR_PARAMS, hipMemcpyHostToDevice);
@@ -859,6 +865,12 @@ int mpm_corrector_get_params(mpm_corrector* c, float* params, int n) {
     if (!c || !params || n != CORR_PARAMS) return cfail(MPM_EINVAL, "parameter count must be MPM_CORRECTOR_PARAMS");
     CHK(hipMemcpyAsync(params, c->st.params, sizeof(float) * CORR_PARAMS, hipMemcpyDeviceToHost, c->stream));
     CHK(hipStreamSynchronize(c->stream));
+    return MPM_OK;
+}
+
+int mpm_corrector_set_deadzone(mpm_corrector* c, float dead) {
+    if (!c || !(dead >= 0.0f)) return cfail(MPM_EINVAL, "dead zone must be >= 0");
+    c->st.out_dead = dead;
     return MPM_OK;
 }
 
@@ -926,7 +938,7 @@ int mpm_corrector_gradient(mpm_corrector* c, int row0, int n, float* grad_out, f
         if ((rc = alloc(&c->loss, 1))) return rc;
         c->loss_cap = 1;
     }
-    Norm nm{c->st.mean, c->st.inv_std, c->st.out_scale, c->st.out_clamp};
+    Norm nm{c->st.mean, c->st.inv_std, c->st.out_scale, c->st.out_clamp, c->st.out_dead};
     hipLaunchKernelGGL(k_zero_f32, dim3((CORR_PARAMS + 255) / 256), dim3(256), 0, c->stream, c->grad, CORR_PARAMS);
     hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, c->stream, c->loss, 1);
     hipLaunchKernelGGL(k_train, dim3(grid_blocks(n, 32, 1024)), dim3(64), 0, c->stream, c->dX, c->dT, row0, n,
@@ -955,7 +967,7 @@ int mpm_corrector_train(mpm_corrector* c, int steps, int batch, float lr, float*
         c->loss_cap = steps;
     }
     hipLaunchKernelGGL(k_zero_f32, dim3((steps + 255) / 256), dim3(256), 0, c->stream, c->loss, steps);
-    Norm nm{c->st.mean, c->st.inv_std, c->st.out_scale, c->st.out_clamp};
+    Norm nm{c->st.mean, c->st.inv_std, c->st.out_scale, c->st.out_clamp, c->st.out_dead};
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     for (int s = 0; s < steps; ++s) {
         const int row0 = (int)(((int64_t)s * batch + c->adam_t * 7919) % c->nrows);

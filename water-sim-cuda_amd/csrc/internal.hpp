@@ -179,6 +179,7 @@ struct CorrState {
     float* inv_std;   // CORR_STRIDE
     float out_scale;  // correction = output * out_scale
     float out_clamp;  // |output| limit applied by the forward pass
+    float out_dead;   // outputs of smaller magnitude give no correction
 };
 
 void corr_pack(const float* params, void* frags, hipStream_t s);

@@ -32,7 +32,8 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--seeds", type=int, default=32, help="training runs (seeds 0..N-1)")
     ap.add_argument("--samples", type=int, default=2_000_000, help="training pairs in all")
-    ap.add_argument("--per-sample", type=int, default=16384, help="rows kept per collection")
+    ap.add_argument("--collections", type=int, default=20,
+                    help="pair collections per seed, spread evenly over the simulated second")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -47,15 +48,18 @@ def main() -> int:
     learned = importlib.import_module("water-sim-cuda_amd.learned")
 
     t0 = time.time()
-    collections = max(1, int(np.ceil(args.samples / args.seeds / args.per_sample)))
-    every = max(learned.RATIO, 2000 // collections)
-    X, T = learned.generate_pairs(range(args.seeds), args.samples, every=every, per_sample=args.per_sample,
+    # pairs at `collections` evenly spaced states of each one-second run, so
+    # that every phase of the flow is in the training set
+    per_sample = int(np.ceil(args.samples / args.seeds / args.collections))
+    every = max(learned.RATIO, 2000 // args.collections)
+    X, T = learned.generate_pairs(range(args.seeds), args.samples, every=every, per_sample=per_sample,
                                   substeps=2000, log=log)
     log(f"pairs: {len(X)} rows in {time.time() - t0:.1f}s; target rms {float(np.sqrt((T ** 2).mean())):.4e}")
     mean, inv, scale = learned.normalisation(X, T)
     corr = m.Corrector(learned.init_params(args.init_seed))
     corr.set_norm(mean, inv, scale)
     corr.set_clamp(learned.OUT_CLAMP)
+    corr.set_deadzone(learned.DEADZONE)
     t1 = time.time()
     losses = learned.train(corr, X, T, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
     t_train = time.time() - t1
@@ -64,6 +68,7 @@ def main() -> int:
     del X, T
     meta = {"family": "dambreak128", "ratio": learned.RATIO, "train_seeds": args.seeds, "samples": args.samples,
             "init": f"learned.init_params(seed={args.init_seed}): He-uniform, output layer x0.1, zero biases",
+            "collections_per_seed": args.collections, "deadzone": learned.DEADZONE, "clamp": learned.OUT_CLAMP,
             "epochs": args.epochs, "batch": args.batch, "lr": args.lr,
             "loss_first": float(losses[:50].mean()), "loss_last": float(losses[-50:].mean()),
             "train_seconds": round(t_train, 2)}
