@@ -57,6 +57,8 @@ constexpr int B_L3 = 60;   // 2 x 8
 constexpr int B_L2 = 76;   // 2 x 8
 constexpr int NFRAG = 92;
 
+// FP32 -> BF16 with round to nearest even: the conversion instruction of
+// gfx950 (v_cvt_pk_bf16_f32, two values per instruction).
 __device__ __forceinline__ uint16_t f2bf(float f) {
     uint32_t u = __float_as_uint(f);
     if ((u & 0x7FFFFFFFu) > 0x7F800000u) return 0x7FC0u;  // NaN
@@ -441,43 +443,49 @@ __global__ __launch_bounds__(256) void k_active_seg(const DevCfg* __restrict__ c
     __shared__ uint32_t wcount[4];
     __shared__ uint32_t wbase;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int i = blockIdx.x * 4 + wave;
-    uint64_t m[4] = {0ull, 0ull, 0ull, 0ull};
-    int x = 0, y = 0, bz = 0, sc = 0;
-    if (i < (int)*nsegs) {
-        const uint2 e = uniform(segs[i]);
-        const int blk = (int)e.x;
-        sc = block_scene[blk];
-        const DevCfg c = uniform(cfgs[sc]);
-        const int lb = blk - c.foff;
-        const int bx = lb % c.fbx, by = (lb / c.fbx) % c.fby;
-        bz = lb / (c.fbx * c.fby);
-        const bool fm = (e.y >> ((lane & 15) >> 2)) & 1u;
-        x = bx * 4 + (lane & 15);
-        y = by * 4 + (lane >> 4);
+    // persistent workgroups: four segments (one per wave) per iteration; the
+    // trip count is the same for the workgroup's waves (barriers inside)
+    const uint32_t ns = *nsegs;
+    for (uint32_t b0 = blockIdx.x * 4u; b0 < ns; b0 += gridDim.x * 4u) {
+        const int i = (int)b0 + wave;
+        uint64_t m[4] = {0ull, 0ull, 0ull, 0ull};
+        int x = 0, y = 0, bz = 0, sc = 0;
+        if (i < (int)ns) {
+            const uint2 e = uniform(segs[i]);
+            const int blk = (int)e.x;
+            sc = block_scene[blk];
+            const DevCfg c = uniform(cfgs[sc]);
+            const int lb = blk - c.foff;
+            const int bx = lb % c.fbx, by = (lb / c.fbx) % c.fby;
+            bz = lb / (c.fbx * c.fby);
+            const bool fm = (e.y >> ((lane & 15) >> 2)) & 1u;
+            x = bx * 4 + (lane & 15);
+            y = by * 4 + (lane >> 4);
+#pragma unroll
+            for (int zz = 0; zz < 4; ++zz) {
+                const int z = bz * 4 + zz;
+                m[zz] = __ballot(fm && inb(c, x, y, z) && V.plane[0][gidx(c, x, y, z)] > 1e-9f);
+            }
+        }
+        const uint32_t mine = (uint32_t)(__popcll(m[0]) + __popcll(m[1]) + __popcll(m[2]) + __popcll(m[3]));
+        if (lane == 0) wcount[wave] = mine;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t tot = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+            wbase = tot ? atomicAdd(count, tot) : 0u;
+        }
+        __syncthreads();
+        uint32_t base = wbase;
+        for (int w = 0; w < wave; ++w) base += wcount[w];
+        const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
 #pragma unroll
         for (int zz = 0; zz < 4; ++zz) {
-            const int z = bz * 4 + zz;
-            m[zz] = __ballot(fm && inb(c, x, y, z) && V.plane[0][gidx(c, x, y, z)] > 1e-9f);
+            const uint32_t slot = base + (uint32_t)__popcll(m[zz] & below);
+            if (((m[zz] >> lane) & 1ull) && slot < (uint32_t)cap)
+                list[slot] = make_int2(x | (y << 10) | ((bz * 4 + zz) << 20), sc);
+            base += (uint32_t)__popcll(m[zz]);
         }
-    }
-    const uint32_t mine = (uint32_t)(__popcll(m[0]) + __popcll(m[1]) + __popcll(m[2]) + __popcll(m[3]));
-    if (lane == 0) wcount[wave] = mine;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t tot = wcount[0] + wcount[1] + wcount[2] + wcount[3];
-        wbase = tot ? atomicAdd(count, tot) : 0u;
-    }
-    __syncthreads();
-    uint32_t base = wbase;
-    for (int w = 0; w < wave; ++w) base += wcount[w];
-    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-#pragma unroll
-    for (int zz = 0; zz < 4; ++zz) {
-        const uint32_t slot = base + (uint32_t)__popcll(m[zz] & below);
-        if (((m[zz] >> lane) & 1ull) && slot < (uint32_t)cap)
-            list[slot] = make_int2(x | (y << 10) | ((bz * 4 + zz) << 20), sc);
-        base += (uint32_t)__popcll(m[zz]);
+        __syncthreads();  // wcount and wbase are reused by the next iteration
     }
 }
 
@@ -504,18 +512,32 @@ __device__ __forceinline__ void node_features(const DevCfg& c, const GridBuf& V,
     float nx, ny, nz;
     node_normal(c, sdf, x, y, z, nx, ny, nz);
     f[5] = nx; f[6] = ny; f[7] = nz;
+    // The 27 neighbours one z slice at a time: a slice's 36 loads are issued
+    // together at coordinates clamped into the grid (one memory latency per
+    // slice rather than per neighbour), and neighbours outside the grid are
+    // then set to zero.
 #pragma unroll
-    for (int o = 0; o < 27; ++o) {
-        const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
-        float mm = 0.0f, pm = 0.0f;
-        if (inb(c, x + dx, y + dy, z + dz)) {
-            const int q = gidx(c, x + dx, y + dy, z + dz);
-            mm = V.plane[0][q];
-            const float vx = V.plane[1][q], vy = V.plane[2][q], vz = V.plane[3][q];
-            pm = mm * sqrtf(vx * vx + vy * vy + vz * vz) * dt;
+    for (int sl = 0; sl < 3; ++sl) {
+        const int zz = z + sl - 1, zc = min(max(zz, 0), c.sz - 1);
+        float mm[9], vx[9], vy[9], vz[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const int xx = x + q % 3 - 1, yy = y + q / 3 - 1;
+            const int gq = gidx(c, min(max(xx, 0), c.sx - 1), min(max(yy, 0), c.sy - 1), zc);
+            mm[q] = V.plane[0][gq];
+            vx[q] = V.plane[1][gq];
+            vy[q] = V.plane[2][gq];
+            vz[q] = V.plane[3][gq];
         }
-        f[8 + o] = mm * 0.125f;
-        f[35 + o] = pm * 0.125f;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const int o = 9 * sl + q;  // x fastest: o % 3 - 1 = dx, (o / 3) % 3 - 1 = dy, o / 9 - 1 = dz
+            const bool in = inb(c, x + q % 3 - 1, y + q / 3 - 1, zz);
+            const float m = in ? mm[q] : 0.0f;
+            const float pm = in ? m * sqrtf(vx[q] * vx[q] + vy[q] * vy[q] + vz[q] * vz[q]) * dt : 0.0f;
+            f[8 + o] = m * 0.125f;
+            f[35 + o] = pm * 0.125f;
+        }
     }
     f[62] = c.gravity * dt * dt;
     f[63] = c.compression * dt * dt;
@@ -715,8 +737,8 @@ void launch_corr_active(const DevCfg* cfgs, const uint16_t* block_scene, GridBuf
 void launch_corr_active_seg(const DevCfg* cfgs, const uint16_t* block_scene, GridBuf V, const uint2* segs,
                             const uint32_t* nsegs, int max_segs, int2* list, int cap, uint32_t* count, hipStream_t s) {
     if (max_segs <= 0) return;
-    hipLaunchKernelGGL(k_active_seg, dim3((max_segs + 3) / 4), dim3(256), 0, s, cfgs, block_scene, V, segs, nsegs,
-                       list, cap, count);
+    hipLaunchKernelGGL(k_active_seg, dim3(grid_blocks(max_segs, 4, 1024)), dim3(256), 0, s, cfgs, block_scene, V,
+                       segs, nsegs, list, cap, count);
 }
 
 void launch_corr_features(const DevCfg* cfgs, GridBuf V, const float* sdf, const int2* list, const uint32_t* count,
@@ -733,13 +755,13 @@ void launch_corr_forward(const CorrState& cs, const float* X, int cap, const uin
 void launch_corr_correct(const DevCfg* cfgs, GridBuf V, const float* sdf, const int2* list, const uint32_t* count,
                          int cap, const CorrState& cs, float* Y, hipStream_t s) {
     Norm nm{cs.mean, cs.inv_std, cs.out_scale, cs.out_clamp, cs.out_dead};
-    hipLaunchKernelGGL(k_correct, dim3(grid_blocks(cap, 128, 1024)), dim3(256), 0, s, cfgs, V, sdf, list, count, cap,
+    hipLaunchKernelGGL(k_correct, dim3(grid_blocks(cap, 256, 512)), dim3(256), 0, s, cfgs, V, sdf, list, count, cap,
                        nm, (const bf16x4*)cs.frags, cs.params, Y);
 }
 
 void launch_corr_apply(const DevCfg* cfgs, GridBuf V, const int2* list, const uint32_t* count, int cap, const float* Y,
                        hipStream_t s) {
-    hipLaunchKernelGGL(k_apply, dim3(grid_blocks(cap, 256, 4096)), dim3(256), 0, s, cfgs, V, list, count, cap, Y);
+    hipLaunchKernelGGL(k_apply, dim3(grid_blocks(cap, 256, 1024)), dim3(256), 0, s, cfgs, V, list, count, cap, Y);
 }
 
 void launch_corr_gather_vel(const DevCfg* cfgs, GridBuf V, const int2* list, const uint32_t* count, int cap, float* out,

@@ -240,8 +240,8 @@ def rms(a: np.ndarray, b: np.ndarray) -> float:
 # on the dam-break family (MI355X): the plain step at DT, and the corrected
 # or uncorrected step at RATIO*DT (particles move RATIO times farther per
 # substep, so they need re-sorting sooner).
-PLAIN_RESORT = 2
-COARSE_RESORT = 1
+PLAIN_RESORT = 4
+COARSE_RESORT = 3
 
 
 def evaluate(corr, seed: int, sim_seconds: float = 1.0, frame_coarse: int = 10, log=None) -> dict:
