@@ -30,6 +30,7 @@ namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int NF = CORR_FEATURES;  // 66 features
 constexpr int KIN = CORR_STRIDE;   // 72: features padded to 9 K-blocks of 8
@@ -57,8 +58,7 @@ constexpr int B_L3 = 60;   // 2 x 8
 constexpr int B_L2 = 76;   // 2 x 8
 constexpr int NFRAG = 92;
 
-// FP32 -> BF16 with round to nearest even: the conversion instruction of
-// gfx950 (v_cvt_pk_bf16_f32, two values per instruction).
+// FP32 -> BF16 with round to nearest even (NaN stays NaN).
 __device__ __forceinline__ uint16_t f2bf(float f) {
     uint32_t u = __float_as_uint(f);
     if ((u & 0x7FFFFFFFu) > 0x7F800000u) return 0x7FC0u;  // NaN
@@ -74,6 +74,21 @@ __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
     r[2] = (short)f2bf(c);
     r[3] = (short)f2bf(d);
     return r;
+}
+
+// The same rounding for values that are not NaN (activations after the ReLU,
+// which maps NaN to 0), two at a time: each value is rounded in place and one
+// byte permute takes the two high halves (2.5 instructions per value instead
+// of the NaN-checked conversion's six).
+__device__ __forceinline__ uint32_t bf_round(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return u + 0x7FFFu + ((u >> 16) & 1u);
+}
+__device__ __forceinline__ bf16x4 pack4_num(float a, float b, float c, float d) {
+    u32x2 r;
+    r[0] = __builtin_amdgcn_perm(bf_round(b), bf_round(a), 0x07060302u);
+    r[1] = __builtin_amdgcn_perm(bf_round(d), bf_round(c), 0x07060302u);
+    return __builtin_bit_cast(bf16x4, r);
 }
 
 __device__ __forceinline__ f32x16 mfma(bf16x4 a, bf16x4 b, f32x16 c) {
@@ -128,39 +143,74 @@ __device__ __forceinline__ void relu_pack(const f32x16 a[2], bf16x4 h[8]) {
 #pragma unroll
     for (int kb = 0; kb < 8; ++kb) {
         const int mb = kb >> 2, q = (kb & 3) * 4;
-        h[kb] = pack4(fmaxf(a[mb][q], 0.0f), fmaxf(a[mb][q + 1], 0.0f), fmaxf(a[mb][q + 2], 0.0f),
-                      fmaxf(a[mb][q + 3], 0.0f));
+        h[kb] = pack4_num(fmaxf(a[mb][q], 0.0f), fmaxf(a[mb][q + 1], 0.0f), fmaxf(a[mb][q + 2], 0.0f),
+                          fmaxf(a[mb][q + 3], 0.0f));
     }
 }
 
+// Layer L's (0..3) bias as the D-layout accumulator of M-block mb, read from
+// the parameters in global memory ...
+struct BiasParams {
+    const float* __restrict__ P;
+    __device__ __forceinline__ f32x16 operator()(int L, int mb, int hi) const {
+        const int off = L == 0 ? P_B1 : L == 1 ? P_B2 : L == 2 ? P_B3 : P_B4;
+        return bias_acc(P + off, mb, hi, L == 3 ? NO : NH);
+    }
+};
+// ... or from a copy in that layout staged once per workgroup in LDS
+// ([layer][M-block][lane half][16], zero padded): four 16-byte reads.
+constexpr int BIAS_D = 4 * 2 * 2 * 16;
+__device__ __forceinline__ void stage_bias(const float* __restrict__ P, float* s) {
+    for (int i = threadIdx.x; i < BIAS_D; i += blockDim.x) {
+        const int L = i >> 6, mb = (i >> 5) & 1, hi = (i >> 4) & 1, r = i & 15;
+        const int u = mb * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+        const int off = L == 0 ? P_B1 : L == 1 ? P_B2 : L == 2 ? P_B3 : P_B4;
+        s[i] = u < (L == 3 ? NO : NH) ? P[off + u] : 0.0f;
+    }
+}
+struct BiasLds {
+    const float* s;
+    __device__ __forceinline__ f32x16 operator()(int L, int mb, int hi) const {
+        const float4* q = reinterpret_cast<const float4*>(s + ((L * 2 + mb) * 2 + hi) * 16);
+        f32x16 a;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float4 v = q[r];
+            a[4 * r] = v.x; a[4 * r + 1] = v.y; a[4 * r + 2] = v.z; a[4 * r + 3] = v.w;
+        }
+        return a;
+    }
+};
+
 // Forward pass of one 32-node tile.  xb: input B fragments (9 K-blocks).
-__device__ __forceinline__ void mlp_fwd(const bf16x4* __restrict__ fr, const float* __restrict__ P,
+template <class Bias>
+__device__ __forceinline__ void mlp_fwd(const bf16x4* __restrict__ fr, Bias bias,
                                         const bf16x4 xb[9], int lane, bf16x4 h1[8], bf16x4 h2[8], bf16x4 h3[8],
                                         f32x16& a4) {
     const int hi = lane >> 5;
     f32x16 a[2];
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
-        a[mb] = bias_acc(P + P_B1, mb, hi, NH);
+        a[mb] = bias(0, mb, hi);
 #pragma unroll
         for (int kb = 0; kb < 9; ++kb) a[mb] = mfma(fr[(F_L1 + mb * 9 + kb) * 64 + lane], xb[kb], a[mb]);
     }
     relu_pack(a, h1);
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
-        a[mb] = bias_acc(P + P_B2, mb, hi, NH);
+        a[mb] = bias(1, mb, hi);
 #pragma unroll
         for (int kb = 0; kb < 8; ++kb) a[mb] = mfma(fr[(F_L2 + mb * 8 + kb) * 64 + lane], h1[kb], a[mb]);
     }
     relu_pack(a, h2);
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
-        a[mb] = bias_acc(P + P_B3, mb, hi, NH);
+        a[mb] = bias(2, mb, hi);
 #pragma unroll
         for (int kb = 0; kb < 8; ++kb) a[mb] = mfma(fr[(F_L3 + mb * 8 + kb) * 64 + lane], h2[kb], a[mb]);
     }
     relu_pack(a, h3);
-    a4 = bias_acc(P + P_B4, 0, hi, NO);
+    a4 = bias(3, 0, hi);
 #pragma unroll
     for (int kb = 0; kb < 8; ++kb) a4 = mfma(fr[(F_L4 + kb) * 64 + lane], h3[kb], a4);
 }
@@ -205,7 +255,9 @@ __global__ __launch_bounds__(256) void k_forward(const float* __restrict__ X, in
                                                  Norm nm, const bf16x4* __restrict__ fr, const float* __restrict__ P,
                                                  float* __restrict__ Y) {
     __shared__ bf16x4 wfr[FWD_FRAGS * 64];
+    __shared__ __align__(16) float bsh[BIAS_D];
     for (int i = threadIdx.x; i < FWD_FRAGS * 64; i += blockDim.x) wfr[i] = fr[i];
+    stage_bias(P, bsh);
     __syncthreads();
     const int rows = count ? (int)min((uint32_t)n, *count) : n;
     const int lane = threadIdx.x & 63, hi = lane >> 5, wave = threadIdx.x >> 6;
@@ -218,7 +270,7 @@ __global__ __launch_bounds__(256) void k_forward(const float* __restrict__ X, in
         // Re-read the weight fragments from LDS in every tile (each is read
         // right before its MFMA) instead of holding all 58 in registers.
         asm volatile("" ::: "memory");
-        mlp_fwd(wfr, P, xb, lane, h1, h2, h3, a4);
+        mlp_fwd(wfr, BiasLds{bsh}, xb, lane, h1, h2, h3, a4);
         if (valid && hi == 0) {
             // guard: outputs beyond the clamp (in target-RMS units, far
             // outside anything the training set holds) are limited
@@ -329,7 +381,7 @@ __global__ __launch_bounds__(64) void k_train(const float* __restrict__ X, const
         bf16x4 xb[9], h1[8], h2[8], h3[8];
         f32x16 a4;
         load_x(X, row, valid, nm.mean, nm.inv_std, hi, xb);
-        mlp_fwd(fr, P, xb, lane, h1, h2, h3, a4);
+        mlp_fwd(fr, BiasParams{P}, xb, lane, h1, h2, h3, a4);
         // output gradient (rows 0..2 live in lanes 0..31, registers 0..2)
         f32x16 g4[2] = {};
         float l2 = 0.0f;
@@ -564,61 +616,69 @@ __global__ __launch_bounds__(256) void k_features(const DevCfg* __restrict__ cfg
 
 // Fused features + forward over the active-node list.  Each wave takes 64
 // nodes at a time, one per lane: the lane computes its node's features and
-// stores them normalised, in BF16, in the first layer's B-fragment order in
-// LDS (node r of tile t: K-block kb's features 8kb..8kb+3 at lane r, 8kb+4..
-// 8kb+7 at lane r + 32); the wave then runs the MLP on its two 32-node tiles.
-// Four waves per workgroup share the forward weight fragments in LDS.
-__global__ __launch_bounds__(256) void k_correct(const DevCfg* __restrict__ cfgs, GridBuf V,
+// packs them, normalised, into BF16 pairs of K-block halves (8kb..8kb+3 and
+// 8kb+4..8kb+7).  The first layer's B operand for node r of a 32-node tile
+// wants the first half at lane r and the second at lane r + 32, so one
+// v_permlane32_swap per register exchanges the upper lanes of the first
+// halves with the lower lanes of the second halves, which leaves tile 0's
+// operand (nodes 0..31) in the one and tile 1's (nodes 32..63) in the other,
+// with no LDS round trip.  Four waves per workgroup share the forward weight
+// fragments, the biases and the normalisation in LDS.
+__global__ __launch_bounds__(256, 3) void k_correct(const DevCfg* __restrict__ cfgs, GridBuf V,
                                                  const float* __restrict__ sdf, const int2* __restrict__ list,
                                                  const uint32_t* __restrict__ count, int cap, Norm nm,
                                                  const bf16x4* __restrict__ fr, const float* __restrict__ P,
                                                  float* __restrict__ Y) {
     __shared__ bf16x4 wfr[FWD_FRAGS * 64];
-    __shared__ bf16x4 xin[4][2][9][64];  // [wave][tile][K-block][lane]
+    __shared__ __align__(16) float bsh[BIAS_D];
+    __shared__ __align__(16) float2 nsh[KIN];  // (1 / std, -mean / std) per feature
     for (int i = threadIdx.x; i < FWD_FRAGS * 64; i += blockDim.x) wfr[i] = fr[i];
+    stage_bias(P, bsh);
+    for (int i = threadIdx.x; i < KIN; i += blockDim.x) nsh[i] = make_float2(nm.inv_std[i], -nm.mean[i] * nm.inv_std[i]);
     __syncthreads();
     const int rows = (int)min(*count, (uint32_t)cap);
-    const int lane = threadIdx.x & 63, hi = lane >> 5, wave = threadIdx.x >> 6;
-    bf16x4 (*xs)[9][64] = xin[wave];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int base = (blockIdx.x * 4 + wave) * 64; base < rows; base += gridDim.x * 4 * 64) {
+        // LDS reads stay in the loop (hoisted, the 144 normalisation terms
+        // would occupy registers for the whole kernel)
+        asm volatile("" ::: "memory");
+        bf16x4 x0[9], x1[9];
         {
             const int row = base + lane;
             float f[KIN];
-            if (row < rows) {
+            const bool live = row < rows;
+            if (live) {
                 const int2 e = list[row];
                 node_features(cfgs[e.y], V, sdf, e.x & 1023, (e.x >> 10) & 1023, e.x >> 20, f);
-            } else {
-#pragma unroll
-                for (int i = 0; i < KIN; ++i) f[i] = nm.mean[i];  // normalises to 0
             }
-            const int t = lane >> 5, r = lane & 31;
 #pragma unroll
-            for (int kb = 0; kb < 9; ++kb)
+            for (int kb = 0; kb < 9; ++kb) {
+                float v[8];
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    float v[4];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const int k = kb * 8 + 4 * h + i;
-                        v[i] = (f[k] - nm.mean[k]) * nm.inv_std[k];
-                    }
-                    xs[t][kb][r + 32 * h] = pack4(v[0], v[1], v[2], v[3]);
+                for (int i = 0; i < 8; ++i) {
+                    const float2 q = nsh[8 * kb + i];
+                    v[i] = live ? fmaf(f[8 * kb + i], q.x, q.y) : 0.0f;  // rows past the end: zero input
                 }
-        }
-        // the wave's LDS operations run in order: the reads below see the writes
-        asm volatile("" ::: "memory");
-        for (int tt = 0; tt < 2; ++tt) {
-            if (base + tt * 32 >= rows) break;
-            bf16x4 xb[9], h1[8], h2[8], h3[8];
+                u32x2 lo = __builtin_bit_cast(u32x2, pack4(v[0], v[1], v[2], v[3]));
+                u32x2 hi = __builtin_bit_cast(u32x2, pack4(v[4], v[5], v[6], v[7]));
 #pragma unroll
-            for (int kb = 0; kb < 9; ++kb) xb[kb] = xs[tt][kb][lane];
-            f32x16 a4;
-            asm volatile("" ::: "memory");  // weight fragments are re-read from LDS per tile
-            mlp_fwd(wfr, P, xb, lane, h1, h2, h3, a4);
-            const int row = base + tt * 32 + (lane & 31);
-            if (row < rows && hi == 0) store_correction(Y, row, a4[0], a4[1], a4[2], nm);
+                for (int d = 0; d < 2; ++d) {
+                    const auto sw = __builtin_amdgcn_permlane32_swap(lo[d], hi[d], false, false);
+                    lo[d] = sw[0];
+                    hi[d] = sw[1];
+                }
+                x0[kb] = __builtin_bit_cast(bf16x4, lo);
+                x1[kb] = __builtin_bit_cast(bf16x4, hi);
+            }
         }
-        asm volatile("" ::: "memory");  // the next nodes overwrite xs
+        const auto tile = [&](const bf16x4 xb[9], int row) {
+            bf16x4 h1[8], h2[8], h3[8];
+            f32x16 a4;
+            mlp_fwd(wfr, BiasLds{bsh}, xb, lane, h1, h2, h3, a4);
+            if (row < rows && lane < 32) store_correction(Y, row, a4[0], a4[1], a4[2], nm);
+        };
+        tile(x0, base + lane);
+        if (base + 32 < rows) tile(x1, base + 32 + lane);
     }
 }
 
