@@ -76,19 +76,27 @@ __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
     return r;
 }
 
-// The same rounding for values that are not NaN (activations after the ReLU,
-// which maps NaN to 0), two at a time: each value is rounded in place and one
-// byte permute takes the two high halves (2.5 instructions per value instead
-// of the NaN-checked conversion's six).
-__device__ __forceinline__ uint32_t bf_round(float f) {
-    const uint32_t u = __float_as_uint(f);
-    return u + 0x7FFFu + ((u >> 16) & 1u);
+// Two FP32 values to a BF16 pair (lo in bits 0..15) on gfx950's conversion
+// instruction (v_cvt_pk_bf16_f32: round to nearest even, NaN kept): one
+// instruction per pair where f2bf above takes six per value.
+__device__ __forceinline__ uint32_t bf_pair(float lo, float hi) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+    const f2 v = {lo, hi};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, b2));
 }
-__device__ __forceinline__ bf16x4 pack4_num(float a, float b, float c, float d) {
+__device__ __forceinline__ bf16x4 pack4_hw(float a, float b, float c, float d) {
     u32x2 r;
-    r[0] = __builtin_amdgcn_perm(bf_round(b), bf_round(a), 0x07060302u);
-    r[1] = __builtin_amdgcn_perm(bf_round(d), bf_round(c), 0x07060302u);
+    r[0] = bf_pair(a, b);
+    r[1] = bf_pair(c, d);
     return __builtin_bit_cast(bf16x4, r);
+}
+// ReLU on the bit pattern: a sign bit makes the integer negative, so the
+// signed median of (bits, 0, +inf) is 0 for negative values, the value for
+// non-negative ones and +inf for NaN (one v_med3_i32; fmaxf(x, 0) takes two,
+// as it quiets NaN first).
+__device__ __forceinline__ float relu(float x) {
+    return __uint_as_float((uint32_t)min(max((int)__float_as_uint(x), 0), 0x7F800000));
 }
 
 __device__ __forceinline__ f32x16 mfma(bf16x4 a, bf16x4 b, f32x16 c) {
@@ -143,8 +151,7 @@ __device__ __forceinline__ void relu_pack(const f32x16 a[2], bf16x4 h[8]) {
 #pragma unroll
     for (int kb = 0; kb < 8; ++kb) {
         const int mb = kb >> 2, q = (kb & 3) * 4;
-        h[kb] = pack4_num(fmaxf(a[mb][q], 0.0f), fmaxf(a[mb][q + 1], 0.0f), fmaxf(a[mb][q + 2], 0.0f),
-                          fmaxf(a[mb][q + 3], 0.0f));
+        h[kb] = pack4_hw(relu(a[mb][q]), relu(a[mb][q + 1]), relu(a[mb][q + 2]), relu(a[mb][q + 3]));
     }
 }
 
@@ -561,9 +568,17 @@ __device__ __forceinline__ void node_features(const DevCfg& c, const GridBuf& V,
     f[2] = V.plane[2][g] * dt;
     f[3] = V.plane[3][g] * dt;
     f[4] = fminf(fmaxf(sdf[g], -4.0f), 4.0f) * 0.25f;
-    float nx, ny, nz;
-    node_normal(c, sdf, x, y, z, nx, ny, nz);
-    f[5] = nx; f[6] = ny; f[7] = nz;
+    // getNormalAtNode's central differences; the hardware square root and
+    // reciprocal (about 1 ulp) replace the correctly rounded ones here, as
+    // for the momentum magnitudes below: these are network inputs rounded
+    // to BF16, where the IEEE sequences' dozen instructions buy nothing.
+    {
+        const float nx = sdf[gidx(c, min(x + 1, c.sx - 1), y, z)] - sdf[gidx(c, max(x - 1, 0), y, z)];
+        const float ny = sdf[gidx(c, x, min(y + 1, c.sy - 1), z)] - sdf[gidx(c, x, max(y - 1, 0), z)];
+        const float nz = sdf[gidx(c, x, y, min(z + 1, c.sz - 1))] - sdf[gidx(c, x, y, max(z - 1, 0))];
+        const float r = __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(nx * nx + ny * ny + nz * nz) + 1e-9f);
+        f[5] = nx * r; f[6] = ny * r; f[7] = nz * r;
+    }
     // The 27 neighbours one z slice at a time: a slice's 36 loads are issued
     // together at coordinates clamped into the grid (one memory latency per
     // slice rather than per neighbour), and neighbours outside the grid are
@@ -586,7 +601,7 @@ __device__ __forceinline__ void node_features(const DevCfg& c, const GridBuf& V,
             const int o = 9 * sl + q;  // x fastest: o % 3 - 1 = dx, (o / 3) % 3 - 1 = dy, o / 9 - 1 = dz
             const bool in = inb(c, x + q % 3 - 1, y + q / 3 - 1, zz);
             const float m = in ? mm[q] : 0.0f;
-            const float pm = in ? m * sqrtf(vx[q] * vx[q] + vy[q] * vy[q] + vz[q] * vz[q]) * dt : 0.0f;
+            const float pm = m * __builtin_amdgcn_sqrtf(vx[q] * vx[q] + vy[q] * vy[q] + vz[q] * vz[q]) * dt;  // 0 where m is
             f[8 + o] = m * 0.125f;
             f[35 + o] = pm * 0.125f;
         }
@@ -644,23 +659,21 @@ __global__ __launch_bounds__(256, 3) void k_correct(const DevCfg* __restrict__ c
         asm volatile("" ::: "memory");
         bf16x4 x0[9], x1[9];
         {
-            const int row = base + lane;
+            // lanes past the end repeat the last node (a node's outputs
+            // depend on its own inputs only, and theirs are not stored)
+            const int2 e = list[min(base + lane, rows - 1)];
             float f[KIN];
-            const bool live = row < rows;
-            if (live) {
-                const int2 e = list[row];
-                node_features(cfgs[e.y], V, sdf, e.x & 1023, (e.x >> 10) & 1023, e.x >> 20, f);
-            }
+            node_features(cfgs[e.y], V, sdf, e.x & 1023, (e.x >> 10) & 1023, e.x >> 20, f);
 #pragma unroll
             for (int kb = 0; kb < 9; ++kb) {
                 float v[8];
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
                     const float2 q = nsh[8 * kb + i];
-                    v[i] = live ? fmaf(f[8 * kb + i], q.x, q.y) : 0.0f;  // rows past the end: zero input
+                    v[i] = fmaf(f[8 * kb + i], q.x, q.y);
                 }
-                u32x2 lo = __builtin_bit_cast(u32x2, pack4(v[0], v[1], v[2], v[3]));
-                u32x2 hi = __builtin_bit_cast(u32x2, pack4(v[4], v[5], v[6], v[7]));
+                u32x2 lo = __builtin_bit_cast(u32x2, pack4_hw(v[0], v[1], v[2], v[3]));
+                u32x2 hi = __builtin_bit_cast(u32x2, pack4_hw(v[4], v[5], v[6], v[7]));
 #pragma unroll
                 for (int d = 0; d < 2; ++d) {
                     const auto sw = __builtin_amdgcn_permlane32_swap(lo[d], hi[d], false, false);
@@ -815,7 +828,7 @@ void launch_corr_forward(const CorrState& cs, const float* X, int cap, const uin
 void launch_corr_correct(const DevCfg* cfgs, GridBuf V, const float* sdf, const int2* list, const uint32_t* count,
                          int cap, const CorrState& cs, float* Y, hipStream_t s) {
     Norm nm{cs.mean, cs.inv_std, cs.out_scale, cs.out_clamp, cs.out_dead};
-    hipLaunchKernelGGL(k_correct, dim3(grid_blocks(cap, 256, 512)), dim3(256), 0, s, cfgs, V, sdf, list, count, cap,
+    hipLaunchKernelGGL(k_correct, dim3(grid_blocks(cap, 256, 768)), dim3(256), 0, s, cfgs, V, sdf, list, count, cap,
                        nm, (const bf16x4*)cs.frags, cs.params, Y);
 }
 
