@@ -22,8 +22,9 @@ boundary.
 `--scene batch64` measures the batched 64-scene workload instead, with its
 block and tile sizes chosen by the autotuner as that workload specifies
 (trials on the warmed-up state, then a CHUNK trial every 10 frames of the
-run; `--no-tune` keeps the hand-set CHUNK); `--graph` replays each frame as a
-HIP graph; `--autotune` and `--tune-online F` tune any scene.
+run; `--no-tune` keeps the hand-set CHUNK); `--scene dambreak128` is tuned
+the same way; `--graph` replays each frame as a HIP graph; `--autotune` and
+`--tune-online F` tune any scene.
 """
 from __future__ import annotations
 
@@ -217,25 +218,28 @@ def main() -> int:
     ap.add_argument("--graph", action="store_true", help="replay each frame's steps() call as a HIP graph")
     ap.add_argument("--frame", type=int, default=0, help="substeps per steps() call [the scene's SUBSTEPS]")
     ap.add_argument("--autotune", action="store_true",
-                    help="pick CHUNK and LAUNCH_PCT by timing trials first [on for batch scenes]")
-    ap.add_argument("--no-tune", action="store_true", help="no autotuning of batch scenes (their hand-set CHUNK)")
+                    help="pick CHUNK and LAUNCH_PCT by timing trials first [on for the batch and dam-break scenes]")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="no autotuning of the batch and dam-break scenes (their hand-set CHUNK)")
     ap.add_argument("--tune-cache", default=None, help="JSON cache file for --autotune")
     ap.add_argument("--tune-online", type=int, default=None, metavar="FRAMES",
                     help="keep tuning CHUNK in the timed run: one trial every FRAMES frames (0: off) "
-                         "[10 for batch scenes, else 0]")
+                         "[10 for the batch and dam-break scenes, else 0]")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "g2p2g_channel256_traffic.txt"),
                     help="rocprofv3-derived HBM bytes per launch of the fused kernel (key: value text)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-corrector evaluation")
     ap.add_argument("--learned-seconds", type=float, default=1.0, help="simulated seconds of the learned evaluation")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the calibration copy of BYTES and print its bandwidth")
+    ap.add_argument("--copy-wide", action="store_true",
+                    help="with --copy-probe: the wide copy (4 loads in flight per thread, nontemporal stores)")
     args = ap.parse_args()
 
     if args.copy_probe:
         mpm = importlib.import_module("water-sim-cuda_amd")
-        ms = mpm.probe_copy(args.copy_probe, 20)
-        print(json.dumps({"copy_bytes": args.copy_probe, "ms_per_copy": ms,
-                          "GBps_read_plus_write": 2 * args.copy_probe / ms / 1e6}), flush=True)
+        ms = mpm.probe_copy(args.copy_probe, 20, wide=args.copy_wide)
+        print(json.dumps({"copy_bytes": args.copy_probe, "kernel": "k_copy_wide" if args.copy_wide else "k_copy",
+                          "ms_per_copy": ms, "GBps_read_plus_write": 2 * args.copy_probe / ms / 1e6}), flush=True)
         return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -259,8 +263,10 @@ def main() -> int:
         grid_desc = [cfg.SIZE_X, cfg.SIZE_Y, cfg.SIZE_Z]
     resort = sc["resort_every"] if args.resort_every is None else args.resort_every
     # the batched workload is defined with block and tile sizes chosen by the
-    # autotuner: tuned first on the warmed-up state, then online during the run
-    if batch and not args.no_tune:
+    # autotuner, and the dam break's splash wants other chunks than its fresh
+    # block: such scenes are tuned first on the warmed-up state, then online
+    # during the run
+    if sc.get("tune") and not args.no_tune:
         args.autotune = True
         if args.tune_online is None:
             args.tune_online = 10

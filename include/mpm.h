@@ -294,6 +294,9 @@ int mpm_sort_keys_noalloc(const float* px, const float* py, const float* pz, int
  * of one copy.  Used to calibrate HBM byte counters and to report the
  * achievable copy bandwidth next to the engine's figures. */
 int mpm_probe_copy(size_t bytes, int iters, double* ms_per_copy);
+/* The same with four 16-byte loads in flight per thread before their stores,
+ * and nontemporal stores: a stronger estimate of the streaming ceiling. */
+int mpm_probe_copy_wide(size_t bytes, int iters, double* ms_per_copy);
 
 /* ---- scene generators (build components; no reference counterpart) ---- */
 

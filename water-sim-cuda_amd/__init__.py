@@ -165,6 +165,7 @@ def load_library() -> ctypes.CDLL:
         "mpm_batch_get_grid": (ctypes.c_int, [eng, ctypes.c_int, c_float_p]),
         "mpm_batch_scene_info": (ctypes.c_int, [eng, ctypes.c_int, c_i64_p]),
         "mpm_probe_copy": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, c_dbl_p]),
+        "mpm_probe_copy_wide": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, c_dbl_p]),
         "mpm_host_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
         "mpm_host_free": (ctypes.c_int, [ctypes.c_void_p]),
         "mpm_device_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
@@ -711,10 +712,12 @@ def pinned_array(shape, dtype=np.float32) -> np.ndarray:
     return np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
 
 
-def probe_copy(nbytes: int, iters: int = 20) -> float:
-    """Mean milliseconds of one device-to-device streaming copy of nbytes."""
+def probe_copy(nbytes: int, iters: int = 20, wide: bool = False) -> float:
+    """Mean milliseconds of one device-to-device streaming copy of nbytes
+    (wide: four 16-byte loads in flight per thread, nontemporal stores)."""
     ms = ctypes.c_double(0.0)
-    _check(load_library().mpm_probe_copy(int(nbytes), int(iters), ctypes.byref(ms)))
+    fn = load_library().mpm_probe_copy_wide if wide else load_library().mpm_probe_copy
+    _check(fn(int(nbytes), int(iters), ctypes.byref(ms)))
     return ms.value
 
 

@@ -1386,7 +1386,7 @@ int mpm_copy_to_host(void* dst, const void* src, size_t bytes) {
     return MPM_OK;
 }
 
-int mpm_probe_copy(size_t bytes, int iters, double* ms_per_copy) {
+static int probe_copy(size_t bytes, int iters, bool wide, double* ms_per_copy) {
     if (bytes < 16 || iters < 1 || !ms_per_copy) return fail(MPM_EINVAL, "bad copy probe arguments");
     const size_t n = bytes / 16;
     float4 *a = nullptr, *b = nullptr;
@@ -1402,9 +1402,9 @@ int mpm_probe_copy(size_t bytes, int iters, double* ms_per_copy) {
     if (err == hipSuccess) err = hipEventCreate(&e1);
     if (err == hipSuccess) err = hipMemsetAsync(a, 0, n * 16, s);
     if (err == hipSuccess) {
-        launch_copy(a, b, n, s);  // warm-up
+        launch_copy(a, b, n, s, wide);  // warm-up
         hipEventRecord(e0, s);
-        for (int i = 0; i < iters; ++i) launch_copy(a, b, n, s);
+        for (int i = 0; i < iters; ++i) launch_copy(a, b, n, s, wide);
         hipEventRecord(e1, s);
         err = hipEventSynchronize(e1);
     }
@@ -1418,6 +1418,12 @@ int mpm_probe_copy(size_t bytes, int iters, double* ms_per_copy) {
     if (err != hipSuccess) return fail(MPM_EHIP, std::string("copy probe: ") + hipGetErrorString(err));
     *ms_per_copy = ms / iters;
     return MPM_OK;
+}
+
+int mpm_probe_copy(size_t bytes, int iters, double* ms_per_copy) { return probe_copy(bytes, iters, false, ms_per_copy); }
+
+int mpm_probe_copy_wide(size_t bytes, int iters, double* ms_per_copy) {
+    return probe_copy(bytes, iters, true, ms_per_copy);
 }
 
 }  // extern "C"

@@ -67,7 +67,7 @@ void launch_make_sdf(const DevCfg& c, const float* tris, int nt, float* sdf, hip
 int launch_make_sdf_bvh(const DevCfg& c, const float* tris_dev, const float* tris_host, int nt, float* sdf,
                         unsigned int* exact_count, hipStream_t s);
 void launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
-void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s);
+void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s, bool wide);
 // Copies `planes` particle planes (n slots) and the caller indices.
 void launch_copy_state(const float* src, size_t src_stride, const int* src_orig, float* dst, size_t dst_stride,
                        int* dst_orig, int n, int planes, hipStream_t s);

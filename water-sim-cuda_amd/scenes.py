@@ -53,14 +53,16 @@ def sphere64(physical: bool = False, seed: int = 1, radius: float = 12.0) -> dic
 def dambreak128(physical: bool = True, seed: int = 2) -> dict:
     """128^3 dam break: 48x96x48-cell block against the x=4 wall (N = 1,769,472),
     UV-sphere collider (32 slices x 33 stacks = 2,048 triangles, r = 20),
-    DT 5e-4, 20 substeps x 200 frames, re-sort every 2 substeps."""
+    DT 5e-4, 20 substeps x 200 frames, re-sort every 4 substeps; the whole
+    run is autotuned (its splash phase wants smaller chunks than the fresh
+    block)."""
     m = _pkg()
     cfg = m.make_config(SIZE_X=128, SIZE_Y=128, SIZE_Z=128, PADDING=3, DT=5e-4, RESOLUTION=1.0, SUBSTEPS=20,
                         SDF_RESOLUTION=128, **_params(physical))
     parts = m.scene_fill("box", (4.0, 4.0, 40.0), (52.0, 100.0, 88.0), seed)
     tris = m.mesh_uv_sphere((88.3, 24.3, 64.3), 20.0, 32, 33)
-    return dict(name="dambreak128", cfg=cfg, particles=parts, sdf=None, triangles=tris, resort_every=2,
-                frames=200, description="128^3 dam break 48x96x48 cells, UV-sphere r=20 (2048 tris)")
+    return dict(name="dambreak128", cfg=cfg, particles=parts, sdf=None, triangles=tris, resort_every=4,
+                frames=200, tune=True, description="128^3 dam break 48x96x48 cells, UV-sphere r=20 (2048 tris)")
 
 
 def channel256(physical: bool = True, seed: int = 3) -> dict:
@@ -176,7 +178,7 @@ def batch_scenes(count: int = 64, seed: int = 4, n_min: int = 5000, n_max: int =
         sdfs.append(sdf)
         counts.append(p.shape[1])
     return dict(name=f"batch{count}", cfgs=cfgs, counts=np.array(counts, dtype=np.int32),
-                particles=np.concatenate(parts, axis=1), sdfs=sdfs, resort_every=4, chunk=256, frames=100,
+                particles=np.concatenate(parts, axis=1), sdfs=sdfs, resort_every=4, chunk=256, frames=100, tune=True,
                 description=f"{count} scenes, edges {edges}, {n_min}-{n_max} particles log-uniform (capped at "
                             f"0.3*8*edge^3), sphere/cube water, floor or sphere collider, DT 1e-3")
 
