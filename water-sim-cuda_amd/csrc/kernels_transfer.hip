@@ -66,49 +66,52 @@ __device__ __forceinline__ void weights(float p, int b, float w[3], float d[3]) 
 
 // ---- G2P (kernels.cu:199-280) ----
 // vel = sum w v and C = 4 sum w v (x_i - x_p)^T over the 27 nodes.  The
-// sums are grouped by axis (z innermost, then y, then x), which needs about
-// 280 multiply-adds instead of the 650 of a node-by-node accumulation; the
-// additions happen in a different order than the reference's.
+// sums are grouped by axis (z innermost, then y, then x).  Per axis the node
+// offset is x_i - x_p = o - f with o = 0, 1, 2 and f = -d[0], so a sum
+// sum_o w_o (o - f) v_o is w_1 v_1 + 2 w_2 v_2 - f sum_o w_o v_o: the o terms
+// take two products instead of three, and the -f vel terms are added once at
+// the end (about 250 multiply-adds; 650 node by node).  The additions happen
+// in a different order than the reference's.
 template <class Fetch>
 __device__ __forceinline__ void g2p_gather(Fetch fetch, const float wx[3], const float wy[3], const float wz[3],
                                            const float dx[3], const float dy[3], const float dz[3], float vel[3],
                                            float C[9]) {
-    float wxd[3], wyd[3], wzd[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        wxd[i] = wx[i] * dx[i];
-        wyd[i] = wy[i] * dy[i];
-        wzd[i] = wz[i] * dz[i];
-    }
+    const float wz2 = 2.0f * wz[2], wy2 = 2.0f * wy[2], wx2 = 2.0f * wx[2];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         float S[3] = {0.f, 0.f, 0.f}, Y[3] = {0.f, 0.f, 0.f}, T[3] = {0.f, 0.f, 0.f};
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            float s[3] = {0.f, 0.f, 0.f}, t[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const float3 v = fetch(i, j, k);
-                s[0] += wz[k] * v.x; s[1] += wz[k] * v.y; s[2] += wz[k] * v.z;
-                t[0] += wzd[k] * v.x; t[1] += wzd[k] * v.y; t[2] += wzd[k] * v.z;
-            }
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                S[a] += wy[j] * s[a];
-                Y[a] += wyd[j] * s[a];
-                T[a] += wy[j] * t[a];
+            const float3 v0 = fetch(i, j, 0), v1 = fetch(i, j, 1), v2 = fetch(i, j, 2);
+            // s = sum_k wz v, t = sum_k wz k v
+            const float s0 = wz[0] * v0.x + wz[1] * v1.x + wz[2] * v2.x;
+            const float s1 = wz[0] * v0.y + wz[1] * v1.y + wz[2] * v2.y;
+            const float s2 = wz[0] * v0.z + wz[1] * v1.z + wz[2] * v2.z;
+            const float t0 = wz[1] * v1.x + wz2 * v2.x;
+            const float t1 = wz[1] * v1.y + wz2 * v2.y;
+            const float t2 = wz[1] * v1.z + wz2 * v2.z;
+            S[0] += wy[j] * s0; S[1] += wy[j] * s1; S[2] += wy[j] * s2;
+            T[0] += wy[j] * t0; T[1] += wy[j] * t1; T[2] += wy[j] * t2;
+            if (j > 0) {
+                const float wyo = j == 1 ? wy[1] : wy2;  // wy_j j
+                Y[0] += wyo * s0; Y[1] += wyo * s1; Y[2] += wyo * s2;
             }
         }
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             vel[a] += wx[i] * S[a];
-            C[3 * a + 0] += wxd[i] * S[a];
+            if (i > 0) C[3 * a + 0] += (i == 1 ? wx[1] : wx2) * S[a];
             C[3 * a + 1] += wx[i] * Y[a];
             C[3 * a + 2] += wx[i] * T[a];
         }
     }
+    const float fx = -dx[0], fy = -dy[0], fz = -dz[0];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) C[i] *= 4.0f;
+    for (int a = 0; a < 3; ++a) {
+        C[3 * a + 0] = 4.0f * (C[3 * a + 0] - fx * vel[a]);
+        C[3 * a + 1] = 4.0f * (C[3 * a + 1] - fy * vel[a]);
+        C[3 * a + 2] = 4.0f * (C[3 * a + 2] - fz * vel[a]);
+    }
 }
 
 // The same sums with the 27 node velocities delivered nine at a time, one
