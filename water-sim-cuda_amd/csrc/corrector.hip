@@ -92,9 +92,11 @@ __device__ __forceinline__ bf16x4 pack4_hw(float a, float b, float c, float d) {
     return __builtin_bit_cast(bf16x4, r);
 }
 // ReLU on the bit pattern: a sign bit makes the integer negative, so the
-// signed median of (bits, 0, +inf) is 0 for negative values, the value for
-// non-negative ones and +inf for NaN (one v_med3_i32; fmaxf(x, 0) takes two,
-// as it quiets NaN first).
+// signed median of (bits, 0, +inf) is 0 for negative values and the value for
+// non-negative ones (one v_med3_i32; fmaxf(x, 0) takes two, as it quiets NaN
+// first).  A NaN becomes +inf (sign bit clear, as the hardware makes them) or
+// 0 (sign bit set); either way the state it came from is already non-finite,
+// which the fallback checks catch.
 __device__ __forceinline__ float relu(float x) {
     return __uint_as_float((uint32_t)min(max((int)__float_as_uint(x), 0), 0x7F800000));
 }
