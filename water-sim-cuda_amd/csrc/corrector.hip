@@ -256,6 +256,18 @@ __device__ __forceinline__ void store_correction(float* __restrict__ Y, int row,
     Y[(size_t)row * 3 + 2] = live ? fminf(fmaxf(o2, -nm.clamp), nm.clamp) * nm.out_scale : 0.0f;
 }
 
+// A node's corrected velocity: v + the correction of store_correction, written
+// to the grid only where the output is outside the dead zone (elsewhere the
+// grid already holds v).
+__device__ __forceinline__ void apply_correction(float* __restrict__ ux, float* __restrict__ uy,
+                                                 float* __restrict__ uz, int g, float vx, float vy, float vz, float o0,
+                                                 float o1, float o2, const Norm& nm) {
+    if (o0 * o0 + o1 * o1 + o2 * o2 < nm.dead * nm.dead) return;
+    ux[g] = vx + fminf(fmaxf(o0, -nm.clamp), nm.clamp) * nm.out_scale;
+    uy[g] = vy + fminf(fmaxf(o1, -nm.clamp), nm.clamp) * nm.out_scale;
+    uz[g] = vz + fminf(fmaxf(o2, -nm.clamp), nm.clamp) * nm.out_scale;
+}
+
 // Forward over rows [0, *count) (or n when count is null): Y[row][3] =
 // out_scale * clamp(MLP(X[row])).  Four waves per workgroup share the forward
 // weight fragments staged in LDS once; each wave loops over 32-row tiles.
@@ -561,14 +573,24 @@ __global__ __launch_bounds__(256) void k_active_seg(const DevCfg* __restrict__ c
 //   62     GRAVITY * DT^2,  63  COMPRESSION * DT^2
 //   64     distance to the nearest padded wall / 8, clamped to [0, 1]
 //   65     y / SIZE_Y
+// With gv, the node's grid index and velocity are returned as well.
+struct NodeGV {
+    int g;
+    float v[3];
+};
 __device__ __forceinline__ void node_features(const DevCfg& c, const GridBuf& V, const float* __restrict__ sdf, int x, int y, int z,
-                              float* __restrict__ f) {
+                              float* __restrict__ f, NodeGV* gv = nullptr) {
     const int g = gidx(c, x, y, z);
     const float dt = c.dt;
+    const float v0 = V.plane[1][g], v1 = V.plane[2][g], v2 = V.plane[3][g];
+    if (gv) {
+        gv->g = g;
+        gv->v[0] = v0; gv->v[1] = v1; gv->v[2] = v2;
+    }
     f[0] = V.plane[0][g] * 0.125f;
-    f[1] = V.plane[1][g] * dt;
-    f[2] = V.plane[2][g] * dt;
-    f[3] = V.plane[3][g] * dt;
+    f[1] = v0 * dt;
+    f[2] = v1 * dt;
+    f[3] = v2 * dt;
     f[4] = fminf(fmaxf(sdf[g], -4.0f), 4.0f) * 0.25f;
     // getNormalAtNode's central differences; the hardware square root and
     // reciprocal (about 1 ulp) replace the correctly rounded ones here, as
@@ -603,7 +625,9 @@ __device__ __forceinline__ void node_features(const DevCfg& c, const GridBuf& V,
             const int o = 9 * sl + q;  // x fastest: o % 3 - 1 = dx, (o / 3) % 3 - 1 = dy, o / 9 - 1 = dz
             const bool in = inb(c, x + q % 3 - 1, y + q / 3 - 1, zz);
             const float m = in ? mm[q] : 0.0f;
-            const float pm = m * __builtin_amdgcn_sqrtf(vx[q] * vx[q] + vy[q] * vy[q] + vz[q] * vz[q]) * dt;  // 0 where m is
+            // (a massless node's velocity may be stale in the corrector's copy)
+            const float pm = m > 0.0f ? m * __builtin_amdgcn_sqrtf(vx[q] * vx[q] + vy[q] * vy[q] + vz[q] * vz[q]) * dt
+                                      : 0.0f;
             f[8 + o] = m * 0.125f;
             f[35 + o] = pm * 0.125f;
         }
@@ -640,12 +664,18 @@ __global__ __launch_bounds__(256) void k_features(const DevCfg* __restrict__ cfg
 // halves with the lower lanes of the second halves, which leaves tile 0's
 // operand (nodes 0..31) in the one and tile 1's (nodes 32..63) in the other,
 // with no LDS round trip.  Four waves per workgroup share the forward weight
-// fragments, the biases and the normalisation in LDS.
-__global__ __launch_bounds__(256, 3) void k_correct(const DevCfg* __restrict__ cfgs, GridBuf V,
+// fragments, the biases and the normalisation in LDS.  The features read the
+// grid's masses and a copy of its velocities (R: planes 1..3 written by the
+// grid update), so the corrected velocities can go straight into the grid's
+// velocity planes ux, uy, uz.  They are distinct from every array read here
+// and declared __restrict__: stored through the grid's plain plane pointers,
+// the compiler allowed for aliasing and spilled about a hundred registers.
+__global__ __launch_bounds__(256, 3) void k_correct(const DevCfg* __restrict__ cfgs, GridBuf R,
+                                                 float* __restrict__ ux, float* __restrict__ uy,
+                                                 float* __restrict__ uz,
                                                  const float* __restrict__ sdf, const int2* __restrict__ list,
                                                  const uint32_t* __restrict__ count, int cap, Norm nm,
-                                                 const bf16x4* __restrict__ fr, const float* __restrict__ P,
-                                                 float* __restrict__ Y) {
+                                                 const bf16x4* __restrict__ fr, const float* __restrict__ P) {
     __shared__ bf16x4 wfr[FWD_FRAGS * 64];
     __shared__ __align__(16) float bsh[BIAS_D];
     __shared__ __align__(16) float2 nsh[KIN];  // (1 / std, -mean / std) per feature
@@ -660,12 +690,13 @@ __global__ __launch_bounds__(256, 3) void k_correct(const DevCfg* __restrict__ c
         // would occupy registers for the whole kernel)
         asm volatile("" ::: "memory");
         bf16x4 x0[9], x1[9];
+        NodeGV gv;  // the lane's node (grid index) and velocity
         {
             // lanes past the end repeat the last node (a node's outputs
             // depend on its own inputs only, and theirs are not stored)
             const int2 e = list[min(base + lane, rows - 1)];
             float f[KIN];
-            node_features(cfgs[e.y], V, sdf, e.x & 1023, (e.x >> 10) & 1023, e.x >> 20, f);
+            node_features(cfgs[e.y], R, sdf, e.x & 1023, (e.x >> 10) & 1023, e.x >> 20, f, &gv);
 #pragma unroll
             for (int kb = 0; kb < 9; ++kb) {
                 float v[8];
@@ -686,28 +717,23 @@ __global__ __launch_bounds__(256, 3) void k_correct(const DevCfg* __restrict__ c
                 x1[kb] = __builtin_bit_cast(bf16x4, hi);
             }
         }
-        const auto tile = [&](const bf16x4 xb[9], int row) {
+        // node r of a tile's outputs sits in lane r: tile 0's nodes are lanes
+        // 0..31's own, tile 1's come from lanes 32..63
+        const auto tile = [&](const bf16x4 xb[9], int row, int g, float vx, float vy, float vz) {
             bf16x4 h1[8], h2[8], h3[8];
             f32x16 a4;
             mlp_fwd(wfr, BiasLds{bsh}, xb, lane, h1, h2, h3, a4);
-            if (row < rows && lane < 32) store_correction(Y, row, a4[0], a4[1], a4[2], nm);
+            if (row < rows && lane < 32) apply_correction(ux, uy, uz, g, vx, vy, vz, a4[0], a4[1], a4[2], nm);
         };
-        tile(x0, base + lane);
-        if (base + 32 < rows) tile(x1, base + 32 + lane);
-    }
-}
-
-// V[list node] += Y[row] (the corrections).
-__global__ void k_apply(const DevCfg* __restrict__ cfgs, GridBuf V, const int2* __restrict__ list,
-                        const uint32_t* __restrict__ count, int cap, const float* __restrict__ Y) {
-    const int n = (int)min(*count, (uint32_t)cap);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int2 e = list[i];
-        const DevCfg& c = cfgs[e.y];
-        const int g = gidx(c, e.x & 1023, (e.x >> 10) & 1023, e.x >> 20);
-        V.plane[1][g] += Y[(size_t)i * 3 + 0];
-        V.plane[2][g] += Y[(size_t)i * 3 + 1];
-        V.plane[3][g] += Y[(size_t)i * 3 + 2];
+        tile(x0, base + lane, gv.g, gv.v[0], gv.v[1], gv.v[2]);
+        if (base + 32 < rows) {
+            const auto upper = [](float v) {
+                return __uint_as_float(__builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false,
+                                                                         false)[1]);
+            };
+            const int g1 = (int)__builtin_amdgcn_permlane32_swap((uint32_t)gv.g, (uint32_t)gv.g, false, false)[1];
+            tile(x1, base + 32 + lane, g1, upper(gv.v[0]), upper(gv.v[1]), upper(gv.v[2]));
+        }
     }
 }
 
@@ -827,16 +853,13 @@ void launch_corr_forward(const CorrState& cs, const float* X, int cap, const uin
                        (const bf16x4*)cs.frags, cs.params, Y);
 }
 
-void launch_corr_correct(const DevCfg* cfgs, GridBuf V, const float* sdf, const int2* list, const uint32_t* count,
-                         int cap, const CorrState& cs, float* Y, hipStream_t s) {
+void launch_corr_correct(const DevCfg* cfgs, GridBuf V, GridBuf vcopy, const float* sdf, const int2* list,
+                         const uint32_t* count, int cap, const CorrState& cs, hipStream_t s) {
     Norm nm{cs.mean, cs.inv_std, cs.out_scale, cs.out_clamp, cs.out_dead};
-    hipLaunchKernelGGL(k_correct, dim3(grid_blocks(cap, 256, 768)), dim3(256), 0, s, cfgs, V, sdf, list, count, cap,
-                       nm, (const bf16x4*)cs.frags, cs.params, Y);
-}
-
-void launch_corr_apply(const DevCfg* cfgs, GridBuf V, const int2* list, const uint32_t* count, int cap, const float* Y,
-                       hipStream_t s) {
-    hipLaunchKernelGGL(k_apply, dim3(grid_blocks(cap, 256, 1024)), dim3(256), 0, s, cfgs, V, list, count, cap, Y);
+    GridBuf R = vcopy;
+    R.plane[0] = V.plane[0];
+    hipLaunchKernelGGL(k_correct, dim3(grid_blocks(cap, 256, 768)), dim3(256), 0, s, cfgs, R, V.plane[1], V.plane[2],
+                       V.plane[3], sdf, list, count, cap, nm, (const bf16x4*)cs.frags, cs.params);
 }
 
 void launch_corr_gather_vel(const DevCfg* cfgs, GridBuf V, const int2* list, const uint32_t* count, int cap, float* out,

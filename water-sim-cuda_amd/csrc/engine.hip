@@ -137,7 +137,8 @@ struct mpm_engine {
     bool pair_mode = false;  // collect_pairs: accumulate fine-step velocities after each grid update
     int2* clist = nullptr;   // active nodes
     float* cX = nullptr;     // features, CORR_STRIDE per node
-    float* cY = nullptr;     // corrections / coarse velocities, 3 per node
+    float* cY = nullptr;     // coarse velocities of collected pairs, 3 per node
+    float* cV = nullptr;     // velocity copy of corrected substeps: 3 planes of G (zeroed once)
     float* cacc = nullptr;   // accumulated fine velocities, 3 per node
     float* chits = nullptr;
     int ccap = 0;
@@ -401,24 +402,29 @@ void run_transfer(mpm_engine* h, int mode, int launch) {
 uint32_t* corr_count(mpm_engine* h) { return h->sc.counters + 12; }
 
 void run_grid_update(mpm_engine* h) {
+    // a corrected substep: the grid update also writes its velocities to a
+    // copy that the corrector's features read while it corrects the grid
+    const bool correct = h->correct && h->corr && h->ccap > 0;
+    GridBuf vcopy{};
+    if (correct)
+        for (int k = 0; k < 3; ++k) vcopy.plane[1 + k] = h->cV + (size_t)k * h->G;
     {
         StageTimer t(h, ST_GRID);
         // the list counters alternate with the grid buffers: [14 + vcur]
         // now, [15 - vcur] for the next update (cleared by this one)
         launch_grid_update(h->d_cfgs, h->d_block_scene, h->S, h->grid[1 - h->vcur], h->grid[h->vcur], true, h->sdf,
                            h->nfb, h->grid_list, h->sc.counters + 14 + h->vcur, h->sc.counters + 15 - h->vcur,
-                           h->stream);
+                           h->stream, correct ? &vcopy : nullptr);
         h->vcur = 1 - h->vcur;
     }
     uint32_t* segs = h->sc.counters + 15 - h->vcur;  // this update's list count
     const GridBuf& V = h->grid[h->vcur];
-    if (h->correct && h->corr && h->ccap > 0) {
+    if (correct) {
         StageTimer t(h, ST_CORR);
         launch_set_u32(corr_count(h), 0, 0u, h->stream);
         launch_corr_active_seg(h->d_cfgs, h->d_block_scene, V, h->grid_list, segs, h->max_segs,
                                h->clist, h->ccap, corr_count(h), h->stream);
-        launch_corr_correct(h->d_cfgs, V, h->sdf, h->clist, corr_count(h), h->ccap, *h->corr, h->cY, h->stream);
-        launch_corr_apply(h->d_cfgs, V, h->clist, corr_count(h), h->ccap, h->cY, h->stream);
+        launch_corr_correct(h->d_cfgs, V, vcopy, h->sdf, h->clist, corr_count(h), h->ccap, *h->corr, h->stream);
     }
     if (h->pair_mode)
         launch_corr_gather_vel(h->d_cfgs, V, h->clist, corr_count(h), h->ccap, h->cacc, h->chits, 2, h->stream);
@@ -768,8 +774,10 @@ int ensure_corr_buffers(mpm_engine* h) {
     const size_t cap = h->G;
     int rc;
     if ((rc = dev_alloc(&h->clist, cap)) || (rc = dev_alloc(&h->cX, cap * CORR_STRIDE)) ||
-        (rc = dev_alloc(&h->cY, cap * 3)) || (rc = dev_alloc(&h->cacc, cap * 3)) || (rc = dev_alloc(&h->chits, cap)))
+        (rc = dev_alloc(&h->cY, cap * 3)) || (rc = dev_alloc(&h->cacc, cap * 3)) || (rc = dev_alloc(&h->chits, cap)) ||
+        (rc = dev_alloc(&h->cV, cap * 3)))
         return rc;
+    HIPCHK(hipMemset(h->cV, 0, cap * 3 * sizeof(float)));
     h->ccap = (int)cap;
     return MPM_OK;
 }
@@ -825,6 +833,7 @@ int mpm_engine_destroy(mpm_engine* h) {
     dev_free(h->clist);
     dev_free(h->cX);
     dev_free(h->cY);
+    dev_free(h->cV);
     dev_free(h->cacc);
     dev_free(h->chits);
     dev_free(h->check_buf);

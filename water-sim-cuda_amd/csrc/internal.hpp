@@ -53,10 +53,11 @@ void launch_transfer(int mode, const TransferArgs& a, int grid_chunks, hipStream
 // entries) and *count are scratch.  With next_count null *count is cleared
 // first; otherwise *count must be zero already and the update clears
 // *next_count for the next one (two counters used in turn need no clearing
-// launch).
+// launch).  With vcopy, the new velocities are also written to its planes
+// 1..3 (a copy the corrector reads while it updates the grid in place).
 void launch_grid_update(const DevCfg* cfgs, const uint16_t* block_scene, int nscenes, GridBuf m, GridBuf v_clear,
                         bool clear_v, const float* sdf, int nblocks, uint2* list, uint32_t* count,
-                        uint32_t* next_count, hipStream_t s);
+                        uint32_t* next_count, hipStream_t s, const GridBuf* vcopy = nullptr);
 
 // makeSDF of one scene (c: host copy of its DevCfg) into its part of `sdf`.
 void launch_make_sdf(const DevCfg& c, const float* tris, int nt, float* sdf, hipStream_t s);
@@ -194,11 +195,11 @@ void launch_corr_active_seg(const DevCfg* cfgs, const uint16_t* block_scene, Gri
 void launch_corr_features(const DevCfg* cfgs, GridBuf V, const float* sdf, const int2* list, const uint32_t* count,
                           int cap, float* X, hipStream_t s);
 void launch_corr_forward(const CorrState& cs, const float* X, int cap, const uint32_t* count, float* Y, hipStream_t s);
-// Features + forward pass of the listed nodes in one kernel: corrections Y.
-void launch_corr_correct(const DevCfg* cfgs, GridBuf V, const float* sdf, const int2* list, const uint32_t* count,
-                         int cap, const CorrState& cs, float* Y, hipStream_t s);
-void launch_corr_apply(const DevCfg* cfgs, GridBuf V, const int2* list, const uint32_t* count, int cap, const float* Y,
-                       hipStream_t s);
+// Features + forward pass of the listed nodes in one kernel, the corrected
+// velocities written into V; the features read V's masses and the velocity
+// copy in planes 1..3 of vcopy (as the grid update left them).
+void launch_corr_correct(const DevCfg* cfgs, GridBuf V, GridBuf vcopy, const float* sdf, const int2* list,
+                         const uint32_t* count, int cap, const CorrState& cs, hipStream_t s);
 // Velocities of the listed nodes: copied (mode 0), accumulated (mode 1) or
 // copied (mode 2); modes 1 and 2 count the calls in which the node had mass.
 void launch_corr_gather_vel(const DevCfg* cfgs, GridBuf V, const int2* list, const uint32_t* count, int cap, float* out,

@@ -31,9 +31,12 @@ __device__ __forceinline__ GuNode grid_update_load(GridBuf m, const float* __res
 }
 
 // fm: the node's block was touched by the last P2G (update it); fv: the
-// node's block of the previous velocity buffer is to be zeroed.
+// node's block of the previous velocity buffer is to be zeroed.  With a
+// velocity copy (planes 1..3 of `vc`), the new velocity is written there too.
+template <bool COPY>
 __device__ __forceinline__ void grid_update_store(const DevCfg& c, GridBuf m, GridBuf v, const float* __restrict__ sdf,
-                                                  const GuNode& n, int x, int y, int z, bool fm, bool fv) {
+                                                  const GuNode& n, int x, int y, int z, bool fm, bool fv,
+                                                  const GridBuf& vc) {
     if (n.g < 0) return;
     const int g = n.g;
     if (fv) {
@@ -66,6 +69,11 @@ __device__ __forceinline__ void grid_update_store(const DevCfg& c, GridBuf m, Gr
     m.plane[1][g] = vx;
     m.plane[2][g] = vy;
     m.plane[3][g] = vz;
+    if (COPY) {
+        vc.plane[1][g] = vx;
+        vc.plane[2][g] = vy;
+        vc.plane[3][g] = vz;
+    }
 }
 
 // The grid update is restricted to the 4^3 node blocks that the last P2G
@@ -125,11 +133,14 @@ __global__ void k_set_u32_grid(uint32_t* p, uint32_t v) { *p = v; }
 
 // 2. One wave per segment at a time: lane = (x 0..15, y 0..3), the four z
 // slices' loads issued together.  single: every block belongs to scene 0.
+// COPY: the velocities also go to planes 1..3 of vc (corrected substeps).
+template <bool COPY>
 __global__ __launch_bounds__(256) void k_grid_update(const DevCfg* __restrict__ cfgs,
                                                      const uint16_t* __restrict__ block_scene, GridBuf m, GridBuf v,
                                                      const float* __restrict__ sdf, const uint2* __restrict__ list,
                                                      const uint32_t* __restrict__ count,
-                                                     uint32_t* __restrict__ next_count, int single, int cap) {
+                                                     uint32_t* __restrict__ next_count, int single, int cap,
+                                                     GridBuf vc) {
     const int n = (int)min(*count, (uint32_t)cap);
     if (next_count && blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0u;  // the next update's list starts empty
     const int lane = threadIdx.x & 63;
@@ -153,7 +164,7 @@ __global__ __launch_bounds__(256) void k_grid_update(const DevCfg* __restrict__ 
             nd[zz] = grid_update_load(m, sdf, g, fm);
         }
 #pragma unroll
-        for (int zz = 0; zz < 4; ++zz) grid_update_store(c, m, v, sdf, nd[zz], x, y, bz * 4 + zz, fm, fv);
+        for (int zz = 0; zz < 4; ++zz) grid_update_store<COPY>(c, m, v, sdf, nd[zz], x, y, bz * 4 + zz, fm, fv, vc);
     }
 }
 
@@ -442,14 +453,19 @@ void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s, bool w
 
 void launch_grid_update(const DevCfg* cfgs, const uint16_t* block_scene, int nscenes, GridBuf m, GridBuf v_clear,
                         bool clear_v, const float* sdf, int nblocks, uint2* list, uint32_t* count,
-                        uint32_t* next_count, hipStream_t s) {
+                        uint32_t* next_count, hipStream_t s, const GridBuf* vcopy) {
     if (nblocks <= 0) return;
     if (!next_count) hipLaunchKernelGGL(k_set_u32_grid, dim3(1), dim3(1), 0, s, count, 0u);
     hipLaunchKernelGGL(k_grid_list, dim3((nblocks + 255) / 256), dim3(256), 0, s, cfgs, block_scene, m.flags,
                        v_clear.flags, clear_v ? 1 : 0, nblocks, list, count);
     const int waves = std::min((nblocks + 3) / 4, 8192);
-    hipLaunchKernelGGL(k_grid_update, dim3((waves + 3) / 4), dim3(256), 0, s, cfgs, block_scene, m, v_clear, sdf,
-                       list, count, next_count, nscenes == 1 ? 1 : 0, nblocks);
+    const int single = nscenes == 1 ? 1 : 0;
+    if (vcopy)
+        hipLaunchKernelGGL(k_grid_update<true>, dim3((waves + 3) / 4), dim3(256), 0, s, cfgs, block_scene, m, v_clear,
+                           sdf, list, count, next_count, single, nblocks, *vcopy);
+    else
+        hipLaunchKernelGGL(k_grid_update<false>, dim3((waves + 3) / 4), dim3(256), 0, s, cfgs, block_scene, m,
+                           v_clear, sdf, list, count, next_count, single, nblocks, GridBuf{});
 }
 
 void launch_make_sdf(const DevCfg& c, const float* tris, int nt, float* sdf, hipStream_t s) {
