@@ -504,64 +504,6 @@ __global__ __launch_bounds__(256) void k_active(const DevCfg* __restrict__ cfgs,
     if (act && slot < (uint32_t)cap) list[slot] = make_int2(x | (y << 10) | (z << 20), sc);
 }
 
-// The same list from the grid update's list of flagged row segments (four
-// x-adjacent blocks; entry {first block, touched mask | ...}), one wave per
-// entry (lane = (x 0..15, y 0..3), four z slices), one list append per
-// workgroup.  Valid right after the grid update that wrote `segs`.
-__global__ __launch_bounds__(256) void k_active_seg(const DevCfg* __restrict__ cfgs,
-                                                    const uint16_t* __restrict__ block_scene, GridBuf V,
-                                                    const uint2* __restrict__ segs,
-                                                    const uint32_t* __restrict__ nsegs, int2* __restrict__ list,
-                                                    int cap, uint32_t* __restrict__ count) {
-    __shared__ uint32_t wcount[4];
-    __shared__ uint32_t wbase;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // persistent workgroups: four segments (one per wave) per iteration; the
-    // trip count is the same for the workgroup's waves (barriers inside)
-    const uint32_t ns = *nsegs;
-    for (uint32_t b0 = blockIdx.x * 4u; b0 < ns; b0 += gridDim.x * 4u) {
-        const int i = (int)b0 + wave;
-        uint64_t m[4] = {0ull, 0ull, 0ull, 0ull};
-        int x = 0, y = 0, bz = 0, sc = 0;
-        if (i < (int)ns) {
-            const uint2 e = uniform(segs[i]);
-            const int blk = (int)e.x;
-            sc = block_scene[blk];
-            const DevCfg c = uniform(cfgs[sc]);
-            const int lb = blk - c.foff;
-            const int bx = lb % c.fbx, by = (lb / c.fbx) % c.fby;
-            bz = lb / (c.fbx * c.fby);
-            const bool fm = (e.y >> ((lane & 15) >> 2)) & 1u;
-            x = bx * 4 + (lane & 15);
-            y = by * 4 + (lane >> 4);
-#pragma unroll
-            for (int zz = 0; zz < 4; ++zz) {
-                const int z = bz * 4 + zz;
-                m[zz] = __ballot(fm && inb(c, x, y, z) && V.plane[0][gidx(c, x, y, z)] > 1e-9f);
-            }
-        }
-        const uint32_t mine = (uint32_t)(__popcll(m[0]) + __popcll(m[1]) + __popcll(m[2]) + __popcll(m[3]));
-        if (lane == 0) wcount[wave] = mine;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const uint32_t tot = wcount[0] + wcount[1] + wcount[2] + wcount[3];
-            wbase = tot ? atomicAdd(count, tot) : 0u;
-        }
-        __syncthreads();
-        uint32_t base = wbase;
-        for (int w = 0; w < wave; ++w) base += wcount[w];
-        const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-#pragma unroll
-        for (int zz = 0; zz < 4; ++zz) {
-            const uint32_t slot = base + (uint32_t)__popcll(m[zz] & below);
-            if (((m[zz] >> lane) & 1ull) && slot < (uint32_t)cap)
-                list[slot] = make_int2(x | (y << 10) | ((bz * 4 + zz) << 20), sc);
-            base += (uint32_t)__popcll(m[zz]);
-        }
-        __syncthreads();  // wcount and wbase are reused by the next iteration
-    }
-}
-
 // Per-node features (66 used of a 72-float row), from the grid after the
 // grid update (mass plane, velocity planes) and the SDF:
 //   0      mass / 8
@@ -833,13 +775,6 @@ void launch_corr_active(const DevCfg* cfgs, const uint16_t* block_scene, GridBuf
     if (nblocks <= 0) return;
     hipLaunchKernelGGL(k_active, dim3((nblocks + 3) / 4), dim3(256), 0, s, cfgs, block_scene, V, nblocks, list, cap,
                        count);
-}
-
-void launch_corr_active_seg(const DevCfg* cfgs, const uint16_t* block_scene, GridBuf V, const uint2* segs,
-                            const uint32_t* nsegs, int max_segs, int2* list, int cap, uint32_t* count, hipStream_t s) {
-    if (max_segs <= 0) return;
-    hipLaunchKernelGGL(k_active_seg, dim3(grid_blocks(max_segs, 4, 1024)), dim3(256), 0, s, cfgs, block_scene, V,
-                       segs, nsegs, list, cap, count);
 }
 
 void launch_corr_features(const DevCfg* cfgs, GridBuf V, const float* sdf, const int2* list, const uint32_t* count,

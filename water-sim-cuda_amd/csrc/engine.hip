@@ -95,8 +95,7 @@ struct mpm_engine {
     int chunk_est = 0;    // chunk count at the last host synchronisation (sizes launches)
 
     GridBuf grid[2] = {};
-    uint2* grid_list = nullptr;  // max_segs: flagged row segments of a grid update
-    int max_segs = 0;            // row segments (four x-adjacent 4^3 blocks) of all scenes
+    uint2* grid_list = nullptr;  // flagged row segments (four x-adjacent 4^3 blocks) of a grid update
     int vcur = 0;  // grid[vcur]: velocities of the last update; grid[1 - vcur]: accumulates
     float* sdf = nullptr;
 
@@ -402,29 +401,31 @@ void run_transfer(mpm_engine* h, int mode, int launch) {
 uint32_t* corr_count(mpm_engine* h) { return h->sc.counters + 12; }
 
 void run_grid_update(mpm_engine* h) {
-    // a corrected substep: the grid update also writes its velocities to a
-    // copy that the corrector's features read while it corrects the grid
+    // a corrected substep: the grid update also lists the active nodes and
+    // copies the velocities, which the corrector's features read while it
+    // corrects the grid in place
     const bool correct = h->correct && h->corr && h->ccap > 0;
-    GridBuf vcopy{};
-    if (correct)
-        for (int k = 0; k < 3; ++k) vcopy.plane[1 + k] = h->cV + (size_t)k * h->G;
+    CorrGridOut co{};
+    if (correct) {
+        for (int k = 0; k < 3; ++k) co.vcopy.plane[1 + k] = h->cV + (size_t)k * h->G;
+        co.active = h->clist;
+        co.acap = h->ccap;
+        co.acount = corr_count(h);
+    }
     {
         StageTimer t(h, ST_GRID);
+        if (correct) launch_set_u32(corr_count(h), 0, 0u, h->stream);
         // the list counters alternate with the grid buffers: [14 + vcur]
         // now, [15 - vcur] for the next update (cleared by this one)
         launch_grid_update(h->d_cfgs, h->d_block_scene, h->S, h->grid[1 - h->vcur], h->grid[h->vcur], true, h->sdf,
                            h->nfb, h->grid_list, h->sc.counters + 14 + h->vcur, h->sc.counters + 15 - h->vcur,
-                           h->stream, correct ? &vcopy : nullptr);
+                           h->stream, correct ? &co : nullptr);
         h->vcur = 1 - h->vcur;
     }
-    uint32_t* segs = h->sc.counters + 15 - h->vcur;  // this update's list count
     const GridBuf& V = h->grid[h->vcur];
     if (correct) {
         StageTimer t(h, ST_CORR);
-        launch_set_u32(corr_count(h), 0, 0u, h->stream);
-        launch_corr_active_seg(h->d_cfgs, h->d_block_scene, V, h->grid_list, segs, h->max_segs,
-                               h->clist, h->ccap, corr_count(h), h->stream);
-        launch_corr_correct(h->d_cfgs, V, vcopy, h->sdf, h->clist, corr_count(h), h->ccap, *h->corr, h->stream);
+        launch_corr_correct(h->d_cfgs, V, co.vcopy, h->sdf, h->clist, corr_count(h), h->ccap, *h->corr, h->stream);
     }
     if (h->pair_mode)
         launch_corr_gather_vel(h->d_cfgs, V, h->clist, corr_count(h), h->ccap, h->cacc, h->chits, 2, h->stream);
@@ -646,7 +647,6 @@ int create_engine(const mpm_sim_config* cfgs_in, int S, mpm_engine** out) {
     if (nfb >= (1ll << 26)) return cleanup(fail(MPM_ELIMIT, "scenes hold 2^26 or more 4^3 blocks in all"));
     h->G = (size_t)G;
     h->nfb = (int)nfb;
-    for (const DevCfg& d : h->dcs) h->max_segs += d.fby * d.fbz * ((d.fbx + 3) / 4);
     if (hipGetDevice(&h->device) != hipSuccess) return cleanup(fail(MPM_EHIP, "no HIP device"));
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(MPM_EHIP, "hipStreamCreate failed"));

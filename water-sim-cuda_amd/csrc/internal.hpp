@@ -53,11 +53,19 @@ void launch_transfer(int mode, const TransferArgs& a, int grid_chunks, hipStream
 // entries) and *count are scratch.  With next_count null *count is cleared
 // first; otherwise *count must be zero already and the update clears
 // *next_count for the next one (two counters used in turn need no clearing
-// launch).  With vcopy, the new velocities are also written to its planes
-// 1..3 (a copy the corrector reads while it updates the grid in place).
+// launch).  With corr (a corrected substep), the new velocities are also
+// written to planes 1..3 of corr->vcopy (a copy the corrector reads while it
+// updates the grid in place) and the updated nodes with mass > 1e-9 are
+// listed in corr->active (*corr->acount zero on entry, at most acap stored).
+struct CorrGridOut {
+    GridBuf vcopy;
+    int2* active;
+    int acap;
+    uint32_t* acount;
+};
 void launch_grid_update(const DevCfg* cfgs, const uint16_t* block_scene, int nscenes, GridBuf m, GridBuf v_clear,
                         bool clear_v, const float* sdf, int nblocks, uint2* list, uint32_t* count,
-                        uint32_t* next_count, hipStream_t s, const GridBuf* vcopy = nullptr);
+                        uint32_t* next_count, hipStream_t s, const CorrGridOut* corr = nullptr);
 
 // makeSDF of one scene (c: host copy of its DevCfg) into its part of `sdf`.
 void launch_make_sdf(const DevCfg& c, const float* tris, int nt, float* sdf, hipStream_t s);
@@ -188,10 +196,6 @@ void corr_pack(const float* params, void* frags, hipStream_t s);
 // the device, at most cap entries stored).
 void launch_corr_active(const DevCfg* cfgs, const uint16_t* block_scene, GridBuf V, int nblocks, int2* list, int cap,
                         uint32_t* count, hipStream_t s);
-// The same list from the grid update's flagged row segments (segs, *nsegs as
-// left by launch_grid_update; max_segs bounds their number).
-void launch_corr_active_seg(const DevCfg* cfgs, const uint16_t* block_scene, GridBuf V, const uint2* segs,
-                            const uint32_t* nsegs, int max_segs, int2* list, int cap, uint32_t* count, hipStream_t s);
 void launch_corr_features(const DevCfg* cfgs, GridBuf V, const float* sdf, const int2* list, const uint32_t* count,
                           int cap, float* X, hipStream_t s);
 void launch_corr_forward(const CorrState& cs, const float* X, int cap, const uint32_t* count, float* Y, hipStream_t s);

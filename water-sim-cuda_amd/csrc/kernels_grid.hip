@@ -133,14 +133,11 @@ __global__ void k_set_u32_grid(uint32_t* p, uint32_t v) { *p = v; }
 
 // 2. One wave per segment at a time: lane = (x 0..15, y 0..3), the four z
 // slices' loads issued together.  single: every block belongs to scene 0.
-// COPY: the velocities also go to planes 1..3 of vc (corrected substeps).
-template <bool COPY>
 __global__ __launch_bounds__(256) void k_grid_update(const DevCfg* __restrict__ cfgs,
                                                      const uint16_t* __restrict__ block_scene, GridBuf m, GridBuf v,
                                                      const float* __restrict__ sdf, const uint2* __restrict__ list,
                                                      const uint32_t* __restrict__ count,
-                                                     uint32_t* __restrict__ next_count, int single, int cap,
-                                                     GridBuf vc) {
+                                                     uint32_t* __restrict__ next_count, int single, int cap) {
     const int n = (int)min(*count, (uint32_t)cap);
     if (next_count && blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0u;  // the next update's list starts empty
     const int lane = threadIdx.x & 63;
@@ -164,7 +161,79 @@ __global__ __launch_bounds__(256) void k_grid_update(const DevCfg* __restrict__ 
             nd[zz] = grid_update_load(m, sdf, g, fm);
         }
 #pragma unroll
-        for (int zz = 0; zz < 4; ++zz) grid_update_store<COPY>(c, m, v, sdf, nd[zz], x, y, bz * 4 + zz, fm, fv, vc);
+        for (int zz = 0; zz < 4; ++zz) grid_update_store<false>(c, m, v, sdf, nd[zz], x, y, bz * 4 + zz, fm, fv, m);
+    }
+}
+
+// The grid update of a corrected substep: the same per-segment work, plus the
+// velocity copy (planes 1..3 of vc) and the list of nodes with mass > 1e-9
+// ({x | y << 10 | z << 20, scene}, at most acap stored) that the corrector
+// runs over.  A workgroup's four waves take four segments per iteration in
+// step, so that the list takes one atomic per workgroup and iteration.
+__global__ __launch_bounds__(256) void k_grid_update_corr(const DevCfg* __restrict__ cfgs,
+                                                          const uint16_t* __restrict__ block_scene, GridBuf m,
+                                                          GridBuf v, const float* __restrict__ sdf,
+                                                          const uint2* __restrict__ list,
+                                                          const uint32_t* __restrict__ count,
+                                                          uint32_t* __restrict__ next_count, int single, int cap,
+                                                          GridBuf vc, int2* __restrict__ active, int acap,
+                                                          uint32_t* __restrict__ acount) {
+    __shared__ uint32_t wcount[4];
+    __shared__ uint32_t wbase;
+    const uint32_t n = min(*count, (uint32_t)cap);
+    if (next_count && blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0u;  // the next update's list starts empty
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    DevCfg c0{};
+    if (single) c0 = uniform(cfgs[0]);
+    // the trip count is the workgroup's (barriers inside)
+    for (uint32_t b0 = blockIdx.x * 4u; b0 < n; b0 += gridDim.x * 4u) {
+        const uint32_t i = b0 + (uint32_t)wave;
+        uint64_t am[4] = {0ull, 0ull, 0ull, 0ull};
+        int x = 0, y = 0, bz = 0, sc = 0;
+        if (i < n) {
+            const uint2 e = uniform(list[i]);
+            const int blk = (int)e.x;
+            sc = single ? 0 : (int)block_scene[blk];
+            const DevCfg c = single ? c0 : uniform(cfgs[sc]);
+            const int lb = blk - c.foff;
+            const int bx = lb % c.fbx, by = (lb / c.fbx) % c.fby;
+            bz = lb / (c.fbx * c.fby);
+            const int k = (lane & 15) >> 2;
+            const bool fm = (e.y >> k) & 1u, fv = (e.y >> (4 + k)) & 1u;
+            x = bx * 4 + (lane & 15);
+            y = by * 4 + (lane >> 4);
+            GuNode nd[4];
+#pragma unroll
+            for (int zz = 0; zz < 4; ++zz) {
+                const int z = bz * 4 + zz;
+                const int g = (fm || fv) && inb(c, x, y, z) ? gidx(c, x, y, z) : -1;
+                nd[zz] = grid_update_load(m, sdf, g, fm);
+            }
+#pragma unroll
+            for (int zz = 0; zz < 4; ++zz) {
+                grid_update_store<true>(c, m, v, sdf, nd[zz], x, y, bz * 4 + zz, fm, fv, vc);
+                am[zz] = __ballot(fm && nd[zz].g >= 0 && nd[zz].mass > 1e-9f);
+            }
+        }
+        const uint32_t mine = (uint32_t)(__popcll(am[0]) + __popcll(am[1]) + __popcll(am[2]) + __popcll(am[3]));
+        if (lane == 0) wcount[wave] = mine;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t tot = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+            wbase = tot ? atomicAdd(acount, tot) : 0u;
+        }
+        __syncthreads();
+        uint32_t base = wbase;
+        for (int w = 0; w < wave; ++w) base += wcount[w];
+        const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+#pragma unroll
+        for (int zz = 0; zz < 4; ++zz) {
+            const uint32_t slot = base + (uint32_t)__popcll(am[zz] & below);
+            if (((am[zz] >> lane) & 1ull) && slot < (uint32_t)acap)
+                active[slot] = make_int2(x | (y << 10) | ((bz * 4 + zz) << 20), sc);
+            base += (uint32_t)__popcll(am[zz]);
+        }
+        __syncthreads();  // wcount and wbase are reused by the next iteration
     }
 }
 
@@ -453,19 +522,20 @@ void launch_copy(const float4* src, float4* dst, size_t n, hipStream_t s, bool w
 
 void launch_grid_update(const DevCfg* cfgs, const uint16_t* block_scene, int nscenes, GridBuf m, GridBuf v_clear,
                         bool clear_v, const float* sdf, int nblocks, uint2* list, uint32_t* count,
-                        uint32_t* next_count, hipStream_t s, const GridBuf* vcopy) {
+                        uint32_t* next_count, hipStream_t s, const CorrGridOut* corr) {
     if (nblocks <= 0) return;
     if (!next_count) hipLaunchKernelGGL(k_set_u32_grid, dim3(1), dim3(1), 0, s, count, 0u);
     hipLaunchKernelGGL(k_grid_list, dim3((nblocks + 255) / 256), dim3(256), 0, s, cfgs, block_scene, m.flags,
                        v_clear.flags, clear_v ? 1 : 0, nblocks, list, count);
     const int waves = std::min((nblocks + 3) / 4, 8192);
     const int single = nscenes == 1 ? 1 : 0;
-    if (vcopy)
-        hipLaunchKernelGGL(k_grid_update<true>, dim3((waves + 3) / 4), dim3(256), 0, s, cfgs, block_scene, m, v_clear,
-                           sdf, list, count, next_count, single, nblocks, *vcopy);
+    if (corr)
+        hipLaunchKernelGGL(k_grid_update_corr, dim3((waves + 3) / 4), dim3(256), 0, s, cfgs, block_scene, m, v_clear,
+                           sdf, list, count, next_count, single, nblocks, corr->vcopy, corr->active, corr->acap,
+                           corr->acount);
     else
-        hipLaunchKernelGGL(k_grid_update<false>, dim3((waves + 3) / 4), dim3(256), 0, s, cfgs, block_scene, m,
-                           v_clear, sdf, list, count, next_count, single, nblocks, GridBuf{});
+        hipLaunchKernelGGL(k_grid_update, dim3((waves + 3) / 4), dim3(256), 0, s, cfgs, block_scene, m, v_clear, sdf,
+                           list, count, next_count, single, nblocks);
 }
 
 void launch_make_sdf(const DevCfg& c, const float* tris, int nt, float* sdf, hipStream_t s) {
