@@ -31,6 +31,8 @@ namespace {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int NF = CORR_FEATURES;  // 66 features
 constexpr int KIN = CORR_STRIDE;   // 72: features padded to 9 K-blocks of 8
@@ -57,6 +59,16 @@ constexpr int B_L4 = 58;   // 2 x 1 (outputs padded to 8)
 constexpr int B_L3 = 60;   // 2 x 8
 constexpr int B_L2 = 76;   // 2 x 8
 constexpr int NFRAG = 92;
+// Inference fragments for gfx950's K = 16 instruction (v_mfma_f32_32x32x16_bf16,
+// 8 BF16 per lane and operand), stored after the NFRAG fragments above:
+// A = W^T blocks (out mb, in kb) of 32 x 16.  Layer 1 reads the features
+// padded to 80 (5 K-blocks).
+constexpr int G_L1 = 0;    // 2 x 5
+constexpr int G_L2 = 10;   // 2 x 4
+constexpr int G_L3 = 18;   // 2 x 4
+constexpr int G_L4 = 26;   // 1 x 4
+constexpr int NFRAG16 = 30;
+constexpr int KIN16 = 80;
 
 // FP32 -> BF16 with round to nearest even (NaN stays NaN).
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -104,6 +116,15 @@ __device__ __forceinline__ float relu(float x) {
 __device__ __forceinline__ f32x16 mfma(bf16x4 a, bf16x4 b, f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
 }
+__device__ __forceinline__ f32x16 mfma16(bf16x8 a, bf16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ bf16x8 pack8_hw(const float v[8]) {
+    u32x4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = bf_pair(v[2 * i], v[2 * i + 1]);
+    return __builtin_bit_cast(bf16x8, r);
+}
 
 // Weight W[in][out] of layer L (1..4) with zero padding.
 __device__ __forceinline__ float w_at(const float* __restrict__ P, int L, int in, int out) {
@@ -136,6 +157,32 @@ __global__ void k_pack(const float* __restrict__ P, bf16x4* __restrict__ frags) 
         v[i] = fwd ? w_at(P, L, k, m) : w_at(P, L, m, k);
     }
     frags[t] = pack4(v[0], v[1], v[2], v[3]);
+}
+
+// The K = 16 fragments.  For 32x32x16, lane l holds A[l%32][8(l/32)+i] and
+// B[8(l/32)+i][l%32] (i = 0..7), with the same D layout as 32x32x8.  Layer 1's
+// K index is the feature.  A hidden layer's B operand is taken from the
+// previous layer's D registers 8(kb%2) .. 8(kb%2)+7 of M-block kb/2 at the same
+// lane, so its K index i of lane half h is the unit
+// 32(kb/2) + 16(kb%2) + 8(i/4) + 4h + i%4, and the weights are packed in that
+// order.
+__device__ __forceinline__ int unit16(int kb, int h, int i) {
+    return 32 * (kb >> 1) + 16 * (kb & 1) + 8 * (i >> 2) + 4 * h + (i & 3);
+}
+__global__ void k_pack16(const float* __restrict__ P, bf16x8* __restrict__ frags) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= NFRAG16 * 64) return;
+    const int f = t >> 6, l = t & 63, hi = l >> 5, lo = l & 31;
+    int L, mb, kb;
+    if (f < G_L2) { L = 1; mb = f / 5; kb = f % 5; }
+    else if (f < G_L3) { L = 2; mb = (f - G_L2) / 4; kb = (f - G_L2) % 4; }
+    else if (f < G_L4) { L = 3; mb = (f - G_L3) / 4; kb = (f - G_L3) % 4; }
+    else { L = 4; mb = 0; kb = f - G_L4; }
+    float v[8];
+    for (int i = 0; i < 8; ++i) v[i] = w_at(P, L, L == 1 ? 16 * kb + 8 * hi + i : unit16(kb, hi, i), mb * 32 + lo);
+    u32x4 r;
+    for (int i = 0; i < 4; ++i) r[i] = (uint32_t)f2bf(v[2 * i]) | ((uint32_t)f2bf(v[2 * i + 1]) << 16);
+    frags[t] = __builtin_bit_cast(bf16x8, r);
 }
 
 // Bias of the D-layout element r of M-block mb.
@@ -222,6 +269,46 @@ __device__ __forceinline__ void mlp_fwd(const bf16x4* __restrict__ fr, Bias bias
     a4 = bias(3, 0, hi);
 #pragma unroll
     for (int kb = 0; kb < 8; ++kb) a4 = mfma(fr[(F_L4 + kb) * 64 + lane], h3[kb], a4);
+}
+
+// The same forward pass on the K = 16 instruction (inference): xb holds the
+// 5 K-blocks of layer 1's B operand.
+template <class Bias>
+__device__ __forceinline__ void mlp_fwd16(const bf16x8* __restrict__ fr, Bias bias, const bf16x8 xb[5], int lane,
+                                          f32x16& a4) {
+    const int hi = lane >> 5;
+    f32x16 a[2];
+    bf16x8 h[4];
+    const auto relu_pack16 = [&]() {
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            float v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = relu(a[kb >> 1][8 * (kb & 1) + i]);
+            h[kb] = pack8_hw(v);
+        }
+    };
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+        a[mb] = bias(0, mb, hi);
+#pragma unroll
+        for (int kb = 0; kb < 5; ++kb) a[mb] = mfma16(fr[(G_L1 + mb * 5 + kb) * 64 + lane], xb[kb], a[mb]);
+    }
+    relu_pack16();
+#pragma unroll
+    for (int L = 1; L < 3; ++L) {
+        const int base = L == 1 ? G_L2 : G_L3;
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb) {
+            a[mb] = bias(L, mb, hi);
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) a[mb] = mfma16(fr[(base + mb * 4 + kb) * 64 + lane], h[kb], a[mb]);
+        }
+        relu_pack16();
+    }
+    a4 = bias(3, 0, hi);
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) a4 = mfma16(fr[(G_L4 + kb) * 64 + lane], h[kb], a4);
 }
 
 // Input fragments of the tile's node `row` (lanes l and l+32 hold the two
@@ -597,11 +684,12 @@ __global__ __launch_bounds__(256) void k_features(const DevCfg* __restrict__ cfg
     }
 }
 
-// Fused features + forward over the active-node list.  Each wave takes 64
-// nodes at a time, one per lane: the lane computes its node's features and
-// packs them, normalised, into BF16 pairs of K-block halves (8kb..8kb+3 and
-// 8kb+4..8kb+7).  The first layer's B operand for node r of a 32-node tile
-// wants the first half at lane r and the second at lane r + 32, so one
+// Fused features + forward over the active-node list, on the K = 16 MFMA.
+// Each wave takes 64 nodes at a time, one per lane: the lane computes its
+// node's features and packs them, normalised, into BF16 K-block halves
+// (16kb..16kb+7 and 16kb+8..16kb+15).  The first layer's B operand for node r
+// of a 32-node tile wants the first half at lane r and the second at lane
+// r + 32, so one
 // v_permlane32_swap per register exchanges the upper lanes of the first
 // halves with the lower lanes of the second halves, which leaves tile 0's
 // operand (nodes 0..31) in the one and tile 1's (nodes 32..63) in the other,
@@ -617,13 +705,14 @@ __global__ __launch_bounds__(256, 3) void k_correct(const DevCfg* __restrict__ c
                                                  float* __restrict__ uz,
                                                  const float* __restrict__ sdf, const int2* __restrict__ list,
                                                  const uint32_t* __restrict__ count, int cap, Norm nm,
-                                                 const bf16x4* __restrict__ fr, const float* __restrict__ P) {
-    __shared__ bf16x4 wfr[FWD_FRAGS * 64];
+                                                 const bf16x8* __restrict__ fr, const float* __restrict__ P) {
+    __shared__ bf16x8 wfr[NFRAG16 * 64];
     __shared__ __align__(16) float bsh[BIAS_D];
-    __shared__ __align__(16) float2 nsh[KIN];  // (1 / std, -mean / std) per feature
-    for (int i = threadIdx.x; i < FWD_FRAGS * 64; i += blockDim.x) wfr[i] = fr[i];
+    __shared__ __align__(16) float2 nsh[KIN16];  // (1 / std, -mean / std) per feature, 0 past KIN
+    for (int i = threadIdx.x; i < NFRAG16 * 64; i += blockDim.x) wfr[i] = fr[i];
     stage_bias(P, bsh);
-    for (int i = threadIdx.x; i < KIN; i += blockDim.x) nsh[i] = make_float2(nm.inv_std[i], -nm.mean[i] * nm.inv_std[i]);
+    for (int i = threadIdx.x; i < KIN16; i += blockDim.x)
+        nsh[i] = i < KIN ? make_float2(nm.inv_std[i], -nm.mean[i] * nm.inv_std[i]) : make_float2(0.f, 0.f);
     __syncthreads();
     const int rows = (int)min(*count, (uint32_t)cap);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -631,40 +720,41 @@ __global__ __launch_bounds__(256, 3) void k_correct(const DevCfg* __restrict__ c
         // LDS reads stay in the loop (hoisted, the 144 normalisation terms
         // would occupy registers for the whole kernel)
         asm volatile("" ::: "memory");
-        bf16x4 x0[9], x1[9];
+        bf16x8 x0[5], x1[5];
         NodeGV gv;  // the lane's node (grid index) and velocity
         {
             // lanes past the end repeat the last node (a node's outputs
             // depend on its own inputs only, and theirs are not stored)
             const int2 e = list[min(base + lane, rows - 1)];
-            float f[KIN];
+            float f[KIN16];
             node_features(cfgs[e.y], R, sdf, e.x & 1023, (e.x >> 10) & 1023, e.x >> 20, f, &gv);
 #pragma unroll
-            for (int kb = 0; kb < 9; ++kb) {
-                float v[8];
+            for (int i = KIN; i < KIN16; ++i) f[i] = 0.0f;
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const float2 q = nsh[8 * kb + i];
-                    v[i] = fmaf(f[8 * kb + i], q.x, q.y);
+            for (int kb = 0; kb < 5; ++kb) {
+                float v[16];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const float2 q = nsh[16 * kb + i];
+                    v[i] = fmaf(f[16 * kb + i], q.x, q.y);
                 }
-                u32x2 lo = __builtin_bit_cast(u32x2, pack4_hw(v[0], v[1], v[2], v[3]));
-                u32x2 hi = __builtin_bit_cast(u32x2, pack4_hw(v[4], v[5], v[6], v[7]));
+                u32x4 lo = __builtin_bit_cast(u32x4, pack8_hw(v));
+                u32x4 hi = __builtin_bit_cast(u32x4, pack8_hw(v + 8));
 #pragma unroll
-                for (int d = 0; d < 2; ++d) {
+                for (int d = 0; d < 4; ++d) {
                     const auto sw = __builtin_amdgcn_permlane32_swap(lo[d], hi[d], false, false);
                     lo[d] = sw[0];
                     hi[d] = sw[1];
                 }
-                x0[kb] = __builtin_bit_cast(bf16x4, lo);
-                x1[kb] = __builtin_bit_cast(bf16x4, hi);
+                x0[kb] = __builtin_bit_cast(bf16x8, lo);
+                x1[kb] = __builtin_bit_cast(bf16x8, hi);
             }
         }
         // node r of a tile's outputs sits in lane r: tile 0's nodes are lanes
         // 0..31's own, tile 1's come from lanes 32..63
-        const auto tile = [&](const bf16x4 xb[9], int row, int g, float vx, float vy, float vz) {
-            bf16x4 h1[8], h2[8], h3[8];
+        const auto tile = [&](const bf16x8 xb[5], int row, int g, float vx, float vy, float vz) {
             f32x16 a4;
-            mlp_fwd(wfr, BiasLds{bsh}, xb, lane, h1, h2, h3, a4);
+            mlp_fwd16(wfr, BiasLds{bsh}, xb, lane, a4);
             if (row < rows && lane < 32) apply_correction(ux, uy, uz, g, vx, vy, vz, a4[0], a4[1], a4[2], nm);
         };
         tile(x0, base + lane, gv.g, gv.v[0], gv.v[1], gv.v[2]);
@@ -768,6 +858,8 @@ int grid_blocks(int n, int per, int cap) { return std::max(1, std::min((n + per 
 
 void corr_pack(const float* P, void* frags, hipStream_t s) {
     hipLaunchKernelGGL(k_pack, dim3((NFRAG * 64 + 255) / 256), dim3(256), 0, s, P, (bf16x4*)frags);
+    hipLaunchKernelGGL(k_pack16, dim3((NFRAG16 * 64 + 255) / 256), dim3(256), 0, s, P,
+                       reinterpret_cast<bf16x8*>(static_cast<char*>(frags) + (size_t)NFRAG * 64 * 8));
 }
 
 void launch_corr_active(const DevCfg* cfgs, const uint16_t* block_scene, GridBuf V, int nblocks, int2* list, int cap,
@@ -793,8 +885,9 @@ void launch_corr_correct(const DevCfg* cfgs, GridBuf V, GridBuf vcopy, const flo
     Norm nm{cs.mean, cs.inv_std, cs.out_scale, cs.out_clamp, cs.out_dead};
     GridBuf R = vcopy;
     R.plane[0] = V.plane[0];
+    const bf16x8* fr16 = reinterpret_cast<const bf16x8*>(static_cast<const char*>(cs.frags) + (size_t)NFRAG * 64 * 8);
     hipLaunchKernelGGL(k_correct, dim3(grid_blocks(cap, 256, 768)), dim3(256), 0, s, cfgs, R, V.plane[1], V.plane[2],
-                       V.plane[3], sdf, list, count, cap, nm, (const bf16x4*)cs.frags, cs.params);
+                       V.plane[3], sdf, list, count, cap, nm, fr16, cs.params);
 }
 
 void launch_corr_gather_vel(const DevCfg* cfgs, GridBuf V, const int2* list, const uint32_t* count, int cap, float* out,
@@ -863,7 +956,7 @@ int mpm_corrector_create(mpm_corrector** out) {
     *out = nullptr;
     mpm_corrector* c = new mpm_corrector();
     int rc;
-    if ((rc = alloc(&c->st.params, CORR_PARAMS)) || (rc = alloc((uint16_t**)&c->st.frags, (size_t)NFRAG * 256)) ||
+    if ((rc = alloc(&c->st.params, CORR_PARAMS)) || (rc = alloc((uint16_t**)&c->st.frags, (size_t)NFRAG * 256 + (size_t)NFRAG16 * 512)) ||
         (rc = alloc(&c->st.mean, KIN)) || (rc = alloc(&c->st.inv_std, KIN)) || (rc = alloc(&c->grad, CORR_PARAMS)) ||
         (rc = alloc(&c->adam_m, CORR_PARAMS)) || (rc = alloc(&c->adam_v, CORR_PARAMS))) {
         mpm_corrector_destroy(c);

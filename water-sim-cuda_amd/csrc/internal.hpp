@@ -183,7 +183,7 @@ constexpr int CORR_PARAMS = 12803;  // 66*64 + 64 + 2*(64*64 + 64) + 64*3 + 3
 
 struct CorrState {
     float* params;    // FP32 master weights
-    void* frags;      // BF16 MFMA operand fragments derived from params
+    void* frags;      // BF16 MFMA operand fragments derived from params (K = 8, then the K = 16 inference set)
     float* mean;      // CORR_STRIDE
     float* inv_std;   // CORR_STRIDE
     float out_scale;  // correction = output * out_scale
