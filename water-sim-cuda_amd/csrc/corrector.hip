@@ -3,9 +3,11 @@
 // A small MLP (66 -> 64 -> 64 -> 64 -> 3, ReLU) maps per-node features of the
 // grid after the grid update to a velocity correction, so that a substep of
 // 4x the DT behaves like four plain substeps.  Weights and activations are
-// BF16, products accumulate in FP32 on the matrix cores
-// (v_mfma_f32_32x32x8_bf16).  Forward, backward and the Adam update are this
-// file's kernels; the CPU oracle (oracle/mlp.py) checks them.
+// BF16, products accumulate in FP32 on the matrix cores: training on
+// v_mfma_f32_32x32x8_bf16, the engine's inference on gfx950's
+// v_mfma_f32_32x32x16_bf16 (half the instructions; see k_pack16).  Forward,
+// backward and the Adam update are this file's kernels; the CPU oracle
+// (oracle/mlp.py) checks them.
 //
 // Matrix layout.  Every layer computes Out^T = W^T In^T with the nodes of a
 // 32-node tile as the MFMA's N dimension: A = W^T (32 output units x 8 inputs),
