@@ -609,8 +609,9 @@ struct NodeGV {
     int g;
     float v[3];
 };
-__device__ __forceinline__ void node_features(const DevCfg& c, const GridBuf& V, const float* __restrict__ sdf, int x, int y, int z,
-                              float* __restrict__ f, NodeGV* gv = nullptr) {
+// Features 0..7: the node's own mass, velocity, SDF and normal.
+__device__ __forceinline__ void features_own(const DevCfg& c, const GridBuf& V, const float* __restrict__ sdf, int x,
+                                             int y, int z, float* __restrict__ f, NodeGV* gv) {
     const int g = gidx(c, x, y, z);
     const float dt = c.dt;
     const float v0 = V.plane[1][g], v1 = V.plane[2][g], v2 = V.plane[3][g];
@@ -634,6 +635,32 @@ __device__ __forceinline__ void node_features(const DevCfg& c, const GridBuf& V,
         const float r = __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(nx * nx + ny * ny + nz * nz) + 1e-9f);
         f[5] = nx * r; f[6] = ny * r; f[7] = nz * r;
     }
+}
+// A neighbour's mass and momentum magnitude * DT from its loaded values: the
+// mass as stored (0 outside the grid), the momentum 0 where the mass is not
+// positive (a massless node's velocity may be stale in the corrector's copy).
+__device__ __forceinline__ float2 nbr_value(bool in, float m, float vx, float vy, float vz, float dt) {
+    const float mi = in ? m : 0.0f;
+    return make_float2(mi, mi > 0.0f ? mi * __builtin_amdgcn_sqrtf(vx * vx + vy * vy + vz * vz) * dt : 0.0f);
+}
+// Features 62..65 and the padding.
+__device__ __forceinline__ void features_tail(const DevCfg& c, int x, int y, int z, float* __restrict__ f) {
+    const float dt = c.dt;
+    f[62] = c.gravity * dt * dt;
+    f[63] = c.compression * dt * dt;
+    const float p = (float)c.pad;
+    float w = fminf(fminf((float)x - p, (float)(c.sx - 1) - p - (float)x),
+                    fminf(fminf((float)y - p, (float)(c.sy - 1) - p - (float)y),
+                          fminf((float)z - p, (float)(c.sz - 1) - p - (float)z)));
+    f[64] = fminf(fmaxf(w * 0.125f, 0.0f), 1.0f);
+    f[65] = (float)y / (float)c.sy;
+#pragma unroll
+    for (int i = NF; i < KIN; ++i) f[i] = 0.0f;
+}
+
+__device__ __forceinline__ void node_features(const DevCfg& c, const GridBuf& V, const float* __restrict__ sdf, int x, int y, int z,
+                              float* __restrict__ f, NodeGV* gv = nullptr) {
+    features_own(c, V, sdf, x, y, z, f, gv);
     // The 27 neighbours one z slice at a time: a slice's 36 loads are issued
     // together at coordinates clamped into the grid (one memory latency per
     // slice rather than per neighbour), and neighbours outside the grid are
@@ -654,26 +681,24 @@ __device__ __forceinline__ void node_features(const DevCfg& c, const GridBuf& V,
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
             const int o = 9 * sl + q;  // x fastest: o % 3 - 1 = dx, (o / 3) % 3 - 1 = dy, o / 9 - 1 = dz
-            const bool in = inb(c, x + q % 3 - 1, y + q / 3 - 1, zz);
-            const float m = in ? mm[q] : 0.0f;
-            // (a massless node's velocity may be stale in the corrector's copy)
-            const float pm = m > 0.0f ? m * __builtin_amdgcn_sqrtf(vx[q] * vx[q] + vy[q] * vy[q] + vz[q] * vz[q]) * dt
-                                      : 0.0f;
-            f[8 + o] = m * 0.125f;
-            f[35 + o] = pm * 0.125f;
+            const float2 nv = nbr_value(inb(c, x + q % 3 - 1, y + q / 3 - 1, zz), mm[q], vx[q], vy[q], vz[q], c.dt);
+            f[8 + o] = nv.x * 0.125f;
+            f[35 + o] = nv.y * 0.125f;
         }
     }
-    f[62] = c.gravity * dt * dt;
-    f[63] = c.compression * dt * dt;
-    const float p = (float)c.pad;
-    float w = fminf(fminf((float)x - p, (float)(c.sx - 1) - p - (float)x),
-                    fminf(fminf((float)y - p, (float)(c.sy - 1) - p - (float)y),
-                          fminf((float)z - p, (float)(c.sz - 1) - p - (float)z)));
-    f[64] = fminf(fmaxf(w * 0.125f, 0.0f), 1.0f);
-    f[65] = (float)y / (float)c.sy;
-#pragma unroll
-    for (int i = NF; i < KIN; ++i) f[i] = 0.0f;
+    features_tail(c, x, y, z, f);
 }
+
+extern "C" __device__ int __ockl_wfred_min_i32(int);
+extern "C" __device__ int __ockl_wfred_max_i32(int);
+
+// Neighbourhood staging of k_correct: a wave's 64 nodes come from one or two
+// z slices of a row segment (16 x 4 nodes), so their 27-neighbourhoods share
+// most nodes.  When the wave's nodes lie in one scene and the box around them
+// (one node of margin) holds at most STAGE_N nodes, the wave loads the box
+// once, coalesced, as (mass, momentum magnitude * DT) pairs into LDS and each
+// lane reads its 27 neighbours there: 8 node loads per lane instead of 108.
+constexpr int STAGE_N = 512;
 
 __global__ __launch_bounds__(256) void k_features(const DevCfg* __restrict__ cfgs, GridBuf V,
                                                   const float* __restrict__ sdf, const int2* __restrict__ list,
@@ -709,6 +734,7 @@ __global__ __launch_bounds__(256, 3) void k_correct(const DevCfg* __restrict__ c
                                                  const uint32_t* __restrict__ count, int cap, Norm nm,
                                                  const bf16x8* __restrict__ fr, const float* __restrict__ P) {
     __shared__ bf16x8 wfr[NFRAG16 * 64];
+    __shared__ float2 stg[4][STAGE_N];  // per wave: the staged neighbourhood
     __shared__ __align__(16) float bsh[BIAS_D];
     __shared__ __align__(16) float2 nsh[KIN16];  // (1 / std, -mean / std) per feature, 0 past KIN
     for (int i = threadIdx.x; i < NFRAG16 * 64; i += blockDim.x) wfr[i] = fr[i];
@@ -728,8 +754,59 @@ __global__ __launch_bounds__(256, 3) void k_correct(const DevCfg* __restrict__ c
             // lanes past the end repeat the last node (a node's outputs
             // depend on its own inputs only, and theirs are not stored)
             const int2 e = list[min(base + lane, rows - 1)];
+            const int x = e.x & 1023, y = (e.x >> 10) & 1023, z = e.x >> 20;
             float f[KIN16];
-            node_features(cfgs[e.y], R, sdf, e.x & 1023, (e.x >> 10) & 1023, e.x >> 20, f, &gv);
+            // the wave's box: [x0 - 1, x1 + 1] x ... (one scene only)
+            const int sc0 = __builtin_amdgcn_readfirstlane(e.y);
+            const int bx0 = __ockl_wfred_min_i32(x) - 1, by0 = __ockl_wfred_min_i32(y) - 1,
+                      bz0 = __ockl_wfred_min_i32(z) - 1;
+            const int dx = __ockl_wfred_max_i32(x) + 2 - bx0, dy = __ockl_wfred_max_i32(y) + 2 - by0,
+                      dz = __ockl_wfred_max_i32(z) + 2 - bz0;
+            if (__ballot(e.y != sc0) == 0ull && dx * dy * dz <= STAGE_N) {
+                const DevCfg c = uniform(cfgs[sc0]);
+                float2* st = stg[wave];
+                const int vol = dx * dy * dz;
+                // all of a lane's box loads first (clamped coordinates), then
+                // the values
+                float mm[STAGE_N / 64], vx[STAGE_N / 64], vy[STAGE_N / 64], vz[STAGE_N / 64];
+                bool in[STAGE_N / 64];
+                // box index i -> (i mod dx, i / dx mod dy, i / (dx dy)) with
+                // float reciprocals: i + 0.5 lies at least 0.5 / 512 from a
+                // multiple of the divisor and the product errs by < 1e-4
+                const float rdx = 1.0f / (float)dx, rdxy = 1.0f / (float)(dx * dy);
+                // rounds wholly past the box load nothing (a uniform bound)
+                const int rounds = (vol + 63) >> 6;
+#pragma unroll
+                for (int k = 0; k < STAGE_N / 64; ++k) {
+                    if (k >= rounds) break;
+                    const int i = min(lane + 64 * k, vol - 1);
+                    const int iz = (int)(((float)i + 0.5f) * rdxy), r = i - iz * dx * dy;
+                    const int iy = (int)(((float)r + 0.5f) * rdx);
+                    const int gx = bx0 + r - iy * dx, gy = by0 + iy, gz = bz0 + iz;
+                    in[k] = inb(c, gx, gy, gz);
+                    const int g = gidx(c, min(max(gx, 0), c.sx - 1), min(max(gy, 0), c.sy - 1),
+                                       min(max(gz, 0), c.sz - 1));
+                    mm[k] = R.plane[0][g];
+                    vx[k] = R.plane[1][g];
+                    vy[k] = R.plane[2][g];
+                    vz[k] = R.plane[3][g];
+                }
+#pragma unroll
+                for (int k = 0; k < STAGE_N / 64; ++k)
+                    if (k < rounds && lane + 64 * k < vol) st[lane + 64 * k] = nbr_value(in[k], mm[k], vx[k], vy[k], vz[k], c.dt);
+                asm volatile("" ::: "memory");  // the wave's LDS writes are read below, in order
+                features_own(c, R, sdf, x, y, z, f, &gv);
+                const int lo = (x - bx0) + dx * ((y - by0) + dy * (z - bz0));
+#pragma unroll
+                for (int o = 0; o < 27; ++o) {
+                    const float2 nv = st[lo + (o % 3 - 1) + dx * ((o / 3) % 3 - 1) + dx * dy * (o / 9 - 1)];
+                    f[8 + o] = nv.x * 0.125f;
+                    f[35 + o] = nv.y * 0.125f;
+                }
+                features_tail(c, x, y, z, f);
+            } else {
+                node_features(cfgs[e.y], R, sdf, x, y, z, f, &gv);
+            }
 #pragma unroll
             for (int i = KIN; i < KIN16; ++i) f[i] = 0.0f;
 #pragma unroll
