@@ -9,7 +9,9 @@ max(largest cell, ceil(count / 64)) rounds: the fewest that keep the base
 cells of each round distinct within 64 lanes.  Both are computed here with
 numpy from the initial positions.  Right after a re-sort no two lanes of a
 round share a base cell, so the first P2G needs no extra claim pass and no
-particle leaves its tile.
+particle leaves its tile; a step() also runs the next substep's P2G (left in
+the grid for the next call), where only particles whose base cell changed can
+lose a claim and only those whose block changed can leave their tile.
 """
 import numpy as np
 import pytest
@@ -41,6 +43,22 @@ def expected_layout(cfg, pos, cap=None):
     return len(blocks), int(np.maximum(max_b, (n_b + 63) // 64).sum())
 
 
+def moved_particles(cfgs, counts, before, after):
+    """(particles whose base cell changed, those whose 8 x 4 x 4 block
+    changed) between two 26 x n buffers of consecutive scenes."""
+    moved = left = lo = 0
+    for cfg, cnt in zip(cfgs, counts):
+        hi = lo + int(cnt)
+        size = np.array([cfg.SIZE_X, cfg.SIZE_Y, cfg.SIZE_Z], dtype=np.int64)[:, None]
+        b0, b1 = (np.clip(np.trunc(x[0:3, lo:hi].astype(np.float32) - np.float32(0.5)).astype(np.int64), 0, size - 1)
+                  for x in (before, after))
+        blk = np.array([[8], [4], [4]])
+        moved += int(np.any(b0 != b1, axis=0).sum())
+        left += int(np.any(b0 // blk != b1 // blk, axis=0).sum())
+        lo = hi
+    return moved, left
+
+
 def test_layout_single_scene(gpu):
     sc = gpu.scenes.sphere64(physical=False)
     p = sc["particles"]
@@ -55,8 +73,13 @@ def test_layout_single_scene(gpu):
     assert got["rounds"] == rounds
     e.step()
     got = e.counters()
-    assert got["claim_losers"] == 0
-    assert got["tile_overflow_particles"] == 0
+    moved, left = moved_particles([sc["cfg"]], [p.shape[1]], p, e.getParticles())
+    say(f"after one step: claim losers {got['claim_losers']} (base cell changed: {moved}), "
+        f"outside their tile {got['tile_overflow_particles']} (block changed: {left})")
+    # of the lanes sharing a base cell at most one kept its cell (the layout
+    # gave each lane of a round its own), so each extra lane is a moved one
+    assert got["claim_losers"] <= moved
+    assert got["tile_overflow_particles"] <= left
     e.close()
 
 
@@ -84,7 +107,8 @@ def test_layout_odd_size_batch(gpu, cap):
     if cap == 2048:
         assert got["rounds"] == rounds
     eb.step()
-    assert eb.counters()["claim_losers"] == 0
+    moved, _ = moved_particles(b["cfgs"], b["counts"], b["particles"], eb.getParticles())
+    assert eb.counters()["claim_losers"] <= moved
     eb.close()
 
 

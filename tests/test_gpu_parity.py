@@ -178,6 +178,67 @@ def test_twenty_substeps_sphere64(gpu, oracle, sphere_phys):
     e.close()
 
 
+def test_call_boundaries_keep_the_substeps(gpu, oracle, sphere_phys):
+    """A steps() call ends with a fused transfer that also runs the next
+    substep's P2G into the grid, and the next call starts from it: calls of
+    any size, and a CHUNK change (a new layout) between calls, give the
+    oracle's 16 substeps."""
+    sc = sphere_phys
+    say("substeps: call boundaries (8+8, 16 x step(), 3+CHUNK+5+8)")
+    ref = oracle.steps(sc["cfg"], oracle_state(oracle, sc), sc["sdf"], 16, nthreads=8)
+    a = engine_for(gpu, sc)
+    a.steps(8)
+    a.steps(8)
+    b = engine_for(gpu, sc)
+    for _ in range(16):
+        b.step()
+    c = engine_for(gpu, sc, CHUNK=2048)
+    c.steps(3)
+    c.set_option("CHUNK", 256)
+    c.steps(5)
+    c.steps(8)
+    for name, e in (("8+8", a), ("16 x step()", b), ("3+CHUNK+5+8", c)):
+        got = e.getParticles()
+        err = max_err(got[0:3], ref[0:3])
+        say(f"  {name}: position error {err:.2e}")
+        assert err < 16 * POS_TOL
+        assert max_err(got[3:6], ref[3:6]) < 16 * VEL_TOL
+        e.close()
+
+
+def test_pending_p2g_dropped_when_state_changes(gpu, oracle, sphere_phys):
+    """The next substep's P2G that a call leaves in the grid is dropped when
+    the particles or DT change outside steps(): after restore() and after
+    load_state() the run continues from the restored particles, and after a
+    DT change the next substep uses the new DT."""
+    sc = sphere_phys
+    say("substeps: pending P2G after restore, load_state and a DT change")
+    s0 = oracle_state(oracle, sc)
+    ref4 = oracle.steps(sc["cfg"], s0, sc["sdf"], 4, nthreads=8)
+    e = engine_for(gpu, sc)
+    e.snapshot()
+    e.steps(6)
+    e.restore()
+    e.steps(4)
+    assert max_err(e.getParticles()[0:3], ref4[0:3]) < 4 * POS_TOL
+    e.steps(3)
+    e.load_state(s0)
+    e.steps(4)
+    assert max_err(e.getParticles()[0:3], ref4[0:3]) < 4 * POS_TOL
+    e.close()
+    cfg2 = gpu.SimConfig.from_buffer_copy(sc["cfg"])
+    cfg2.DT = sc["cfg"].DT * 2.0
+    ref = oracle.steps(cfg2, oracle.steps(sc["cfg"], s0, sc["sdf"], 3, nthreads=8), sc["sdf"], 3, nthreads=8)
+    f = engine_for(gpu, sc)
+    f.steps(3)
+    f.set_dt_scale(2.0)
+    f.steps(3)
+    err = max_err(f.getParticles()[0:3], ref[0:3])
+    say(f"  DT change: position error {err:.2e}")
+    assert err < 6 * POS_TOL
+    f.close()
+
+
 def test_fused_and_single_steps_agree(gpu, sphere_phys):
     """steps(k) runs the fused G2P+P2G kernel; k calls of step() do not."""
     say("substeps: fused vs single")

@@ -88,6 +88,10 @@ struct mpm_engine {
     // reads positions and F through sc.map from plane set gather_src.
     bool gather_pending = false;
     int gather_src = 0;
+    // The last transfer of a steps() call also ran the next substep's P2G
+    // into grid[1 - vcur] (XFER_G2P2G_FULL); the next call starts from it.
+    // Anything that changes the particles or DT outside steps() discards it.
+    bool p2g_pending = false;
 
     ChunkDesc* chunks = nullptr;
     int chunk_limit = 0;  // bound on the chunk count (sum over scenes of min(n_s, blocks_s) + n / CHUNK + 1)
@@ -119,6 +123,7 @@ struct mpm_engine {
         int k = 0;
         int64_t phase = 0;  // substeps - sorted_at at the start
         int vcur = 0, pcur = 0;
+        bool pending = false;  // p2g_pending at the start (always set at the end)
         int launch = 0;     // transfer launch size baked in
         // end state
         int64_t end_phase = 0;
@@ -390,7 +395,8 @@ TransferArgs make_args(mpm_engine* h) {
 }
 
 void run_transfer(mpm_engine* h, int mode, int launch) {
-    const int stage = mode == XFER_P2G ? ST_P2G : (mode == XFER_G2P ? ST_G2P : ST_G2P2G);
+    // the full fused transfer ends a steps() call: timed with the final G2Ps
+    const int stage = mode == XFER_P2G ? ST_P2G : (mode == XFER_G2P2G ? ST_G2P2G : ST_G2P);
     StageTimer t(h, stage);
     // a pending gather is always consumed by a G2P (resort() keeps it so); a
     // P2G before it reads through the map too
@@ -437,6 +443,15 @@ int zero_grid(mpm_engine* h, int which) {
     return MPM_OK;
 }
 
+// Drops the next substep's P2G that the last steps() call left in
+// grid[1 - vcur] (the particles or DT changed since): that buffer is zeroed
+// again, as a P2G expects it.
+int discard_pending_p2g(mpm_engine* h) {
+    if (!h->p2g_pending) return MPM_OK;
+    h->p2g_pending = false;
+    return zero_grid(h, 1 - h->vcur);
+}
+
 int check_engine(mpm_engine* h, bool need_particles) {
     if (!h) return fail(MPM_EINVAL, "null engine handle");
     if (need_particles && !h->have_particles) return fail(MPM_ESTATE, "particles not initialised (call mpm_engine_init_particles)");
@@ -450,7 +465,9 @@ int check_scene(mpm_engine* h, int scene) {
 }
 
 // Enqueues k substeps (engine.cu:54-70 per substep) and advances the host-side
-// schedule state.  Launches only: safe under stream capture.
+// schedule state.  Launches only: safe under stream capture.  The first P2G
+// is skipped when the previous call left it in the grid; the last transfer
+// writes all particle attributes and runs the next substep's P2G.
 int enqueue_steps(mpm_engine* h, int k, int launch) {
     int rc;
     launch_set_u32(h->overflow, 0, 0u, h->stream);
@@ -459,17 +476,20 @@ int enqueue_steps(mpm_engine* h, int k, int launch) {
         if (s == 0) {
             // A re-sort due now waits for the first fused substep when there
             // is one (reading all attributes through the map would cost the
-            // P2G more than one substep's drift).
+            // P2G more than one substep's drift).  The grid a pending P2G left
+            // does not depend on the layout.
             if (k == 1 && h->substeps - h->sorted_at >= h->resort_every) {
                 if ((rc = resort(h, false))) return rc;
                 h->sorted_at = h->substeps;
             }
-            run_transfer(h, XFER_P2G, launch);
+            if (!h->p2g_pending) run_transfer(h, XFER_P2G, launch);
+            h->p2g_pending = false;
         }
         run_grid_update(h);
         if (s == k - 1) {
-            run_transfer(h, XFER_G2P, launch);
+            run_transfer(h, XFER_G2P2G_FULL, launch);
             h->substeps++;
+            h->p2g_pending = true;
         } else {
             run_transfer(h, XFER_G2P2G, launch);
             h->substeps++;
@@ -493,7 +513,9 @@ int steps_graph(mpm_engine* h, int k) {
     // recently used slot
     mpm_engine::GraphEntry* slot = nullptr;
     for (auto& c : h->graphs)
-        if (c.exec && c.k == k && c.phase == phase && c.vcur == h->vcur && c.pcur == h->pcur) slot = &c;
+        if (c.exec && c.k == k && c.phase == phase && c.vcur == h->vcur && c.pcur == h->pcur &&
+            c.pending == h->p2g_pending)
+            slot = &c;
     if (!slot) {
         slot = &h->graphs[0];
         for (auto& c : h->graphs)
@@ -501,6 +523,7 @@ int steps_graph(mpm_engine* h, int k) {
     }
     mpm_engine::GraphEntry& g = *slot;
     const bool hit = g.exec && g.k == k && g.phase == phase && g.vcur == h->vcur && g.pcur == h->pcur &&
+                     g.pending == h->p2g_pending &&
                      want <= g.launch && want * 2 >= g.launch;
     if (!hit) {
         if (g.exec) hipGraphExecDestroy(g.exec);
@@ -510,6 +533,7 @@ int steps_graph(mpm_engine* h, int k) {
         e.phase = phase;
         e.vcur = h->vcur;
         e.pcur = h->pcur;
+        e.pending = h->p2g_pending;
         e.launch = want + want / 8 + 8;  // headroom for chunk-count growth
         const int64_t sub0 = h->substeps, sorted0 = h->sorted_at;
         HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
@@ -524,6 +548,7 @@ int steps_graph(mpm_engine* h, int k) {
             h->sorted_at = sorted0;
             h->vcur = e.vcur;
             h->pcur = e.pcur;
+            h->p2g_pending = e.pending;
             return rc ? rc : fail(MPM_EHIP, std::string("graph capture: ") + hipGetErrorString(ce));
         }
         hipError_t ie = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
@@ -535,6 +560,7 @@ int steps_graph(mpm_engine* h, int k) {
         h->sorted_at = sorted0;
         h->vcur = e.vcur;
         h->pcur = e.pcur;
+        h->p2g_pending = e.pending;
         if (ie != hipSuccess) return fail(MPM_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
         g = e;
         ++h->graph_captures;
@@ -546,6 +572,7 @@ int steps_graph(mpm_engine* h, int k) {
     h->sorted_at = h->substeps - g.end_phase;
     h->vcur = g.end_vcur;
     h->pcur = g.end_pcur;
+    h->p2g_pending = true;  // every steps(k >= 1) call ends with the next P2G in the grid
     return MPM_OK;
 }
 
@@ -702,6 +729,7 @@ int init_particles_impl(mpm_engine* h, const int* counts, const float* soa26, bo
     for (int b = 0; b < 2; ++b)
         if ((rc = zero_grid(h, b))) return rc;
     h->vcur = 0;
+    h->p2g_pending = false;  // both grids were zeroed above
     // both grid-update list counters start at zero with vcur = 0
     HIPCHK(hipMemsetAsync(h->sc.counters + 14, 0, 2 * sizeof(uint32_t), h->stream));
     h->n = n;
@@ -783,6 +811,10 @@ int ensure_corr_buffers(mpm_engine* h) {
 }
 
 int upload_dt(mpm_engine* h, float scale) {
+    bool same = true;
+    for (int s = 0; s < h->S; ++s) same = same && h->dcs[s].dt == h->cfgs[s].DT * scale;
+    int rc;
+    if (!same && (rc = discard_pending_p2g(h))) return rc;  // its pressure term used the old DT
     for (int s = 0; s < h->S; ++s) h->dcs[s].dt = h->cfgs[s].DT * scale;
     HIPCHK(hipMemcpyAsync(h->d_cfgs, h->dcs.data(), (size_t)h->S * sizeof(DevCfg), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -1090,6 +1122,7 @@ int mpm_engine_collect_pairs(mpm_engine* h, int ratio, float* X_out, float* T_ou
     if (h->n == 0) return MPM_OK;
     if ((rc = ensure_corr_buffers(h))) return rc;
     drop_graph(h);
+    if ((rc = discard_pending_p2g(h))) return rc;  // its own coarse P2G goes into that buffer
     const bool was_correct = h->correct;
     const float scale0 = h->dt_scale;
     h->correct = false;
@@ -1166,6 +1199,7 @@ int mpm_engine_restore(mpm_engine* h) {
     if (h->n == 0) return MPM_OK;
     if (h->snap_substeps < 0) return fail(MPM_ESTATE, "no snapshot taken");
     drop_graph(h);
+    if ((rc = discard_pending_p2g(h))) return rc;
     launch_copy_state(h->snap_planes, (size_t)h->n, h->snap_orig, h->planes[h->pcur], h->plane_cap[h->pcur],
                       h->orig[h->pcur], h->n, A_STATE, h->stream);
     HIPCHK(hipGetLastError());
@@ -1315,6 +1349,7 @@ int mpm_engine_debug_p2g(mpm_engine* h, float* grid4_out) {
     int rc = check_engine(h, true);
     if (rc) return rc;
     if (!grid4_out) return fail(MPM_EINVAL, "null output");
+    if ((rc = discard_pending_p2g(h))) return rc;
     const int m = 1 - h->vcur;
     if (h->n > 0) run_transfer(h, XFER_P2G, transfer_launch_size(h));
     if ((rc = copy_grid_out(h, m, 0, h->G, grid4_out, h->G))) return rc;
@@ -1325,6 +1360,7 @@ int mpm_engine_debug_grid_update(mpm_engine* h, const float* in, float* out) {
     int rc = check_engine(h, false);
     if (rc) return rc;
     if (!in || !out) return fail(MPM_EINVAL, "null argument");
+    if ((rc = discard_pending_p2g(h))) return rc;
     const int m = 1 - h->vcur;
     for (int p = 0; p < 4; ++p)
         HIPCHK(hipMemcpyAsync(h->grid[m].plane[p], in + p * h->G, h->G * sizeof(float), hipMemcpyHostToDevice, h->stream));
@@ -1339,6 +1375,7 @@ int mpm_engine_debug_g2p(mpm_engine* h, const float* in) {
     int rc = check_engine(h, true);
     if (rc) return rc;
     if (!in) return fail(MPM_EINVAL, "null argument");
+    if ((rc = discard_pending_p2g(h))) return rc;  // the particles change below
     const int v = h->vcur;
     for (int p = 0; p < 4; ++p)
         HIPCHK(hipMemcpyAsync(h->grid[v].plane[p], in + p * h->G, h->G * sizeof(float), hipMemcpyHostToDevice, h->stream));

@@ -22,7 +22,9 @@ struct GridBuf {
     uint8_t* flags;  // one byte per 4^3 node block: touched by the last P2G into this buffer
 };
 
-enum TransferMode : int { XFER_P2G = 2, XFER_G2P = 1, XFER_G2P2G = 3 };
+// XFER_G2P2G_FULL: the fused transfer that also writes v and C, so that the
+// particles are complete while the next substep's P2G is already in the grid.
+enum TransferMode : int { XFER_P2G = 2, XFER_G2P = 1, XFER_G2P2G = 3, XFER_G2P2G_FULL = 7 };
 
 struct TransferArgs {
     const DevCfg* cfgs;   // per-scene configs (device table), indexed by ChunkDesc::scene

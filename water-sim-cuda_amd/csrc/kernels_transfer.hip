@@ -26,7 +26,11 @@
 //   XFER_G2P2G : G2P of substep s followed, in registers, by P2G of substep
 //                s+1 into the other grid buffer.  Between substeps only
 //                position and F then travel through HBM; v and C stay in
-//                registers (they are written out by the final XFER_G2P).
+//                registers (they are written out by the last transfer of a
+//                steps() call).
+//   XFER_G2P2G_FULL : the same, also writing v and C: the last transfer of a
+//                steps() call, after which the particles are complete and the
+//                next call starts from the P2G already in the grid.
 #include "internal.hpp"
 
 namespace mpm {
@@ -348,6 +352,7 @@ template <int MODE, bool GATHER>
 __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a) {
     constexpr bool DO_G2P = (MODE & XFER_G2P) != 0;
     constexpr bool DO_P2G = (MODE & XFER_P2G) != 0;
+    constexpr bool WRITE_VC = !DO_P2G || MODE == XFER_G2P2G_FULL;  // G2P also stores v and C
     // Workgroups are dispatched to the 8 XCDs round-robin; chunk ranges are
     // split into 8 contiguous parts so that neighbouring chunks (which share
     // tile nodes) run on the same XCD and share its L2.
@@ -505,7 +510,7 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
                 (a.part + (size_t)A_PZ * a.stride)[slot] = p.z;
 #pragma unroll
                 for (int i = 0; i < 9; ++i) (a.part + (size_t)(A_F0 + i) * a.stride)[slot] = p.F[i];
-                if constexpr (!DO_P2G) {
+                if constexpr (WRITE_VC) {
                     (a.part + (size_t)A_VX * a.stride)[slot] = p.vx;
                     (a.part + (size_t)A_VY * a.stride)[slot] = p.vy;
                     (a.part + (size_t)A_VZ * a.stride)[slot] = p.vz;
@@ -639,6 +644,10 @@ void launch_transfer(int mode, const TransferArgs& a, int grid_chunks, hipStream
         case XFER_G2P:
             if (g) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_transfer<XFER_G2P, true>), grid, block, 0, s, a);
             else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_transfer<XFER_G2P, false>), grid, block, 0, s, a);
+            break;
+        case XFER_G2P2G_FULL:
+            if (g) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_transfer<XFER_G2P2G_FULL, true>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_transfer<XFER_G2P2G_FULL, false>), grid, block, 0, s, a);
             break;
         default:
             if (g) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_transfer<XFER_G2P2G, true>), grid, block, 0, s, a);
