@@ -179,7 +179,10 @@ int mpm_engine_get_sdf(mpm_engine* h, float* sdf_out);
 
 /* One substep: sort (per RESORT_EVERY), clear, P2G, grid update, G2P. */
 int mpm_engine_step(mpm_engine* h);
-/* k substeps back to back (Simulation::run's inner loop, Simulation.cpp:46-49). */
+/* k substeps back to back (Simulation::run's inner loop, Simulation.cpp:46-49).
+ * Each call ends with the particles complete and the next substep's P2G
+ * already in the grid, which the next call starts from (restore, load_state,
+ * a DT change and the debug transfers drop it). */
 int mpm_engine_steps(mpm_engine* h, int k);
 /* Blocks until the engine's stream is idle. */
 int mpm_engine_synchronize(mpm_engine* h);
@@ -187,7 +190,8 @@ int mpm_engine_synchronize(mpm_engine* h);
 /* Saves the particle state (positions, velocities, C, F, substep count) in
  * device memory, and restores it, re-laid out for the current options (no
  * reference counterpart: used by the autotuner to rewind its trials).  The
- * grid needs no saving: each steps() call starts with a P2G of the particles. */
+ * grid needs no saving: after a restore the next steps() call starts with a
+ * P2G of the restored particles. */
 int mpm_engine_snapshot(mpm_engine* h);
 int mpm_engine_restore(mpm_engine* h);
 
@@ -224,11 +228,13 @@ int mpm_engine_get_grid(mpm_engine* h, float* grid4_out);
 
 /* Per-stage device time accumulated while TIMING is on, in milliseconds, and the
  * number of launches: out[2*s] = ms, out[2*s+1] = count for stage s in
- * 0 = P2G, 1 = G2P, 2 = fused G2P+P2G, 3 = grid update, 4 = re-sort.
+ * 0 = P2G, 1 = the last transfer of a call (fused G2P+P2G also writing v and
+ * C), 2 = fused G2P+P2G, 3 = grid update, 4 = re-sort.
  * Resets the accumulators. */
 int mpm_engine_get_stage_times(mpm_engine* h, double* out, int nstages);
 
-/* Diagnostic counters of the last mpm_engine_steps call: out[0] = chunks,
+/* Diagnostic counters of the last mpm_engine_steps call (its transfers, the
+ * P2G it leaves for the next call included): out[0] = chunks,
  * out[1] = particle-substeps whose stencil had left their chunk's LDS tile
  * (global-memory path), out[2] = active 4^3 grid blocks, out[3] =
  * particle-substeps that shared a base node with another lane of their round
