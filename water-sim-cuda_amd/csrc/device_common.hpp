@@ -86,6 +86,7 @@ struct ChunkDesc {
     int S;           // rounds
     int scene;       // scene of the chunk (index into the DevCfg table)
     int block;       // its chunk block when the block is one chunk, else -1
+    int paired;      // 1: a round may hold two members of a cell, in lanes 2k and 2k + 1
 };
 
 // Chunk block: 8 x 4 x 4 base cells, i.e. two x-adjacent 4^3 flag blocks

@@ -410,6 +410,7 @@ __global__ __launch_bounds__(256) void k_bucket_chunks(uint32_t* __restrict__ bc
         d.oy = 4 * by;
         d.oz = 4 * bz;
         d.S = 0;  // set by the reorder
+        d.paired = 0;
         d.scene = scene;
         d.block = pieces == 1 ? b : -1;
         chunks[c] = d;
@@ -449,6 +450,8 @@ static_assert(RO_ITEMS * RO_THREADS == CHUNK_CAP_MAX, "every chunk position has 
 // Dynamic LDS: 3 x cap words (cap <= 2048: chunk positions take 16 bits in
 // the list entries).
 constexpr int RO_SORT_MAX = 64;
+// A chunk takes the paired layout when it needs this share fewer rounds.
+constexpr uint32_t PAIR_SAVE_PCT = 30u;
 template <bool PLANES>
 __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ ch,
                                                         const uint32_t* __restrict__ counters, int cap,
@@ -587,8 +590,16 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
         // block's previous round count when that is at most one more: a
         // different S re-deals every list entry (entry q goes to round q % S),
         // so the next transfer would read the chunk's particles in scattered
-        // order, while an unchanged S moves them by whole rounds.
-        const uint32_t s_min = max(cmax, (n + 63u) / 64u);
+        // order, while an unchanged S moves them by whole rounds.  Where the
+        // fullest cell sets the count (dense cells in a sparse block), a round
+        // may hold two members of a cell instead, which the transfer pairs in
+        // adjacent lanes (one arena update for both): taken when it saves at
+        // least PAIR_SAVE_PCT % of the rounds, as a paired round costs about a
+        // third more.
+        const uint32_t s_one = max(cmax, (n + 63u) / 64u);
+        const uint32_t s_two = max((cmax + 1u) / 2u, (n + 63u) / 64u);
+        const bool paired = 100u * s_two <= (100u - PAIR_SAVE_PCT) * s_one;
+        const uint32_t s_min = paired ? s_two : s_one;
         const int blk = __builtin_amdgcn_readfirstlane(ch[ci].block);
         uint32_t S = s_min;
         if (blk >= 0) {
@@ -597,12 +608,27 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
         }
         const uint32_t nl = (n + S - 1u) / S;     // lanes of the first rounds (<= 64 as S >= n / 64)
         const uint32_t full = n - (nl - 1u) * S;  // rounds with nl lanes, 1 .. S
-        // one wave per round: lane L holds list entry L * S + r
+        // one wave per round: lane L holds list entry L * S + r, so two
+        // members of a cell in one round (entries q and q + S) are in lanes L
+        // and L + 1
         for (uint32_t r = (uint32_t)wave; r < S; r += RO_THREADS / 64) {
             const uint32_t m = r < full ? nl : nl - 1u;
             const bool act = (uint32_t)lane < m;
             const uint32_t e = act ? lst[(uint32_t)lane * S + r] : 0u;
             const uint32_t c = e >> 16;
+            // pairs go first, to lanes 2k and 2k + 1 in their order; then the
+            // single members (only paired chunks have pairs)
+            bool first = false, second = false;
+            uint64_t fm = 0ull;
+            uint32_t pairs_before = 0u;
+            if (paired) {
+                const uint32_t c_next = (uint32_t)__shfl_down((int)c, 1), c_prev = (uint32_t)__shfl_up((int)c, 1);
+                first = act && (uint32_t)lane + 1u < m && c_next == c;
+                second = act && lane > 0 && c_prev == c;
+                fm = __ballot(first);
+                pairs_before = (uint32_t)__popcll(fm & lanemask_lt(lane));
+            }
+            const bool single = act && !first && !second;
             const uint32_t cx = ((c >> 1) & 1u) | (((c >> 4) & 1u) << 1) | ((c & 1u) << 2);
             const uint32_t cy = ((c >> 2) & 1u) | (((c >> 5) & 1u) << 1);
             const uint32_t cz = ((c >> 3) & 1u) | (((c >> 6) & 1u) << 1);
@@ -610,18 +636,22 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
             uint32_t cg[8], rank = 0u;
 #pragma unroll
             for (uint32_t h = 0; h < 8; ++h) {
-                const uint64_t mk = __ballot(act && g == h);
+                const uint64_t mk = __ballot(single && g == h);
                 cg[h] = (uint32_t)__popcll(mk);
                 if (g == h) rank = (uint32_t)__popcll(mk & lanemask_lt(lane));
             }
-            // members of lower rank in any group, then lower groups of the same rank
-            uint32_t pos = 0u;
+            // singles: members of lower rank in any group, then lower groups of
+            // the same rank
+            uint32_t pos = 2u * (uint32_t)__popcll(fm);
 #pragma unroll
             for (uint32_t h = 0; h < 8; ++h) pos += min(cg[h], rank) + ((h < g && cg[h] > rank) ? 1u : 0u);
+            if (first) pos = 2u * pairs_before;
+            if (second) pos = 2u * pairs_before - 1u;  // lane - 1 is the pair's first, counted before
             if (act) cell_rank[e & 0xFFFFu] = (r < full ? r * nl : full * nl + (r - full) * (nl - 1u)) + pos;
         }
         if (tid == 0) {
             ch[ci].S = (int)S;
+            ch[ci].paired = paired ? 1 : 0;
             rsum += S;
             if (blk >= 0) prev_rounds[blk] = (uint16_t)min(S, 65535u);
         }

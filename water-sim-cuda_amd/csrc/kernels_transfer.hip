@@ -336,6 +336,13 @@ __device__ __forceinline__ void lds_add4(float4* slot, const float4 v) {
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
 
+// The value of lane l ^ 1 (a DPP quad permutation [1, 0, 3, 2]); every lane
+// must be active.
+__device__ __forceinline__ int swap_adjacent(int v) { return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false); }
+__device__ __forceinline__ float swap_adjacent(float v) {
+    return __int_as_float(swap_adjacent(__float_as_int(v)));
+}
+
 __device__ __forceinline__ bool node_in_tile(int tx, int ty, int tz) {
     return (unsigned)tx < (unsigned)TILE_DX && (unsigned)ty < (unsigned)TILE_DY && (unsigned)tz < (unsigned)TILE_DZ;
 }
@@ -576,12 +583,52 @@ __global__ __launch_bounds__(TRANSFER_THREADS, 3) void k_transfer(TransferArgs a
                 });
             };
             n_outside += (uint32_t)__popcll(__ballot(valid && !in_tile));
-            if (in_tile) claim[li] = (uint8_t)lane;
+            // Same-cell pairs: in a paired chunk lanes 2k and 2k + 1 can hold
+            // two members of one base cell (k_reorder); the high lane's
+            // contributions reach the low lane through a cross-lane move and
+            // the low lane's chain adds both, so the high lane takes no part
+            // in the claims.
+            bool lo = false, hi = false;
+            if (d.paired) {
+                // the cross-lane moves run with every lane active: no
+                // short-circuit puts them under a branch
+                const int key = in_tile ? (int)li : -1 - lane;
+                const int other = swap_adjacent(key);
+                const bool same = in_tile & (other == key);
+                lo = same & ((lane & 1) == 0);
+                hi = same & ((lane & 1) != 0);
+            }
+            const bool any_pair = __ballot(lo) != 0ull;
+            const bool rmw = in_tile && !hi;
+            if (rmw) claim[li] = (uint8_t)lane;
             asm volatile("" ::: "memory");
-            const bool mine = in_tile && claim[li] == (uint8_t)lane;
+            const bool mine = rmw && claim[li] == (uint8_t)lane;
             asm volatile("" ::: "memory");
-            if (mine) arena_chain(p, pq);
-            const bool lost = in_tile && !mine;
+            bool covered = mine;
+            if (any_pair) {
+                // a pair's high lane is covered when its low lane won the claim
+                const int partner_mine = swap_adjacent(mine ? 1 : 0);
+                covered = mine | (hi & (partner_mine != 0));
+                float4* base = ar + li;
+                p2g_scatter(c, p, pq, [&](int i, int jj, int k, const P2GNode& nd) {
+                    float4 v = nd.value();
+                    const float4 o = make_float4(swap_adjacent(v.x), swap_adjacent(v.y), swap_adjacent(v.z),
+                                                 swap_adjacent(v.w));
+                    if (mine) {
+                        if (lo) {
+                            v.x += o.x; v.y += o.y; v.z += o.z; v.w += o.w;
+                        }
+                        float4* n = base + (i + TILE_PY * jj + TILE_PZ * k);
+                        float4 u = *n;
+                        u.x += v.x; u.y += v.y; u.z += v.z; u.w += v.w;
+                        *n = u;
+                    }
+                    asm volatile("" ::: "memory");  // keep the 27 updates in program order
+                });
+            } else if (mine) {
+                arena_chain(p, pq);
+            }
+            const bool lost = in_tile && !covered;
             n_lost += (uint32_t)__popcll(__ballot(lost));
             if (lost) {
                 Particle p2 = p;
