@@ -450,8 +450,11 @@ static_assert(RO_ITEMS * RO_THREADS == CHUNK_CAP_MAX, "every chunk position has 
 // Dynamic LDS: 3 x cap words (cap <= 2048: chunk positions take 16 bits in
 // the list entries).
 constexpr int RO_SORT_MAX = 64;
-// A chunk takes the paired layout when it needs this share fewer rounds.
-constexpr uint32_t PAIR_SAVE_PCT = 30u;
+// A chunk takes the paired layout when it needs this share (in %) fewer rounds.
+#ifndef MPM_PAIR_SAVE_PCT
+#define MPM_PAIR_SAVE_PCT 30
+#endif
+constexpr uint32_t PAIR_SAVE_PCT = MPM_PAIR_SAVE_PCT;
 template <bool PLANES>
 __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ ch,
                                                         const uint32_t* __restrict__ counters, int cap,
