@@ -483,13 +483,25 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
     // the next chunk's old slots, keys and caller indices, position tid + RO_THREADS i
     uint32_t nsi[RO_ITEMS] = {}, nkey[RO_ITEMS] = {}, norg[RO_ITEMS] = {};
     uint32_t si[RO_ITEMS];  // the current chunk's old slots
-    int nstart = 0, ncount = 0;
-    // (start, count) of chunk c, (0, 0) past the last chunk; read two chunks
-    // ahead so that no chunk waits for its descriptor
-    auto desc_of = [&](uint32_t c) { return c < nch ? make_int2(ch[c].start, ch[c].count) : make_int2(0, 0); };
-    auto load_slots = [&](int2 d) {
+    int nstart = 0, ncount = 0, nblk = -1;
+    uint32_t nprev = 0u;  // the next chunk's block's previous round count (used when nblk >= 0)
+    // (start, count, block) of chunk c, (0, 0, -1) past the last chunk; read
+    // two chunks ahead so that no chunk waits for its descriptor
+    auto desc_of = [&](uint32_t c) {
+        return c < nch ? make_int3(ch[c].start, ch[c].count, ch[c].block) : make_int3(0, 0, -1);
+    };
+    auto load_slots = [&](int3 d) {
         nstart = __builtin_amdgcn_readfirstlane(d.x);
         ncount = __builtin_amdgcn_readfirstlane(d.y);
+        nblk = __builtin_amdgcn_readfirstlane(d.z);
+        // loaded with the slots (unconditionally, block 0 standing in for
+        // none), so that no wait for it drains the loads issued after it
+        // (memory loads complete in order); the index passes through an
+        // empty asm so that the compiler does not move the value to a scalar
+        // register (and wait for it) right after the load
+        int zero = 0;
+        asm volatile("" : "+v"(zero));
+        nprev = (uint32_t)prev_rounds[max(nblk, 0) + zero];
         // Unconditional loads (positions past the chunk read entry 0, which
         // exists as n > 0): a load under a branch would be copied into the
         // loop-carried register, and the copy waits for the load.
@@ -512,17 +524,24 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
     };
     if (tid < BLOCK_CELLS_N) cnt[tid] = 0u;
     load_slots(desc_of(blockIdx.x));
-    int2 dnext = desc_of(blockIdx.x + gridDim.x);
+    int3 dnext = desc_of(blockIdx.x + gridDim.x);
     load_orig();
+    // the first chunk's values waited for here, so that the wait at the top
+    // of the loop only concerns the loads made during the previous chunk
+#pragma unroll
+    for (int i = 0; i < RO_ITEMS; ++i) asm volatile("" ::"v"(nsi[i]), "v"(nkey[i]), "v"(norg[i]));
+    asm volatile("" ::"v"(nprev));
     __syncthreads();
     for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
-        const int start = nstart, count = ncount;
+        const int start = nstart, count = ncount, blk = nblk;
+        const uint32_t prev = __builtin_amdgcn_readfirstlane(nprev);
         // Wait for the loaded values on every path (an empty asm reading
         // them): otherwise the compiler treats their loads as possibly still
         // in flight after the guarded uses below and waits for the next
         // chunk's loads when it reuses these registers.
 #pragma unroll
         for (int i = 0; i < RO_ITEMS; ++i) asm volatile("" ::"v"(nsi[i]), "v"(nkey[i]), "v"(norg[i]));
+        asm volatile("" ::"v"(nprev));
 #pragma unroll
         for (int i = 0; i < RO_ITEMS; ++i) {
             const int q = tid + RO_THREADS * i;
@@ -603,12 +622,8 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
         const uint32_t s_two = max((cmax + 1u) / 2u, (n + 63u) / 64u);
         const bool paired = 100u * s_two <= (100u - PAIR_SAVE_PCT) * s_one;
         const uint32_t s_min = paired ? s_two : s_one;
-        const int blk = __builtin_amdgcn_readfirstlane(ch[ci].block);
         uint32_t S = s_min;
-        if (blk >= 0) {
-            const uint32_t prev = prev_rounds[blk];
-            if (prev >= s_min && prev <= s_min + 1u) S = prev;
-        }
+        if (blk >= 0 && prev >= s_min && prev <= s_min + 1u) S = prev;
         const uint32_t nl = (n + S - 1u) / S;     // lanes of the first rounds (<= 64 as S >= n / 64)
         const uint32_t full = n - (nl - 1u) * S;  // rounds with nl lanes, 1 .. S
         // one wave per round: lane L holds list entry L * S + r, so two
