@@ -86,22 +86,33 @@ __device__ __forceinline__ void grid_update_store(const DevCfg& c, GridBuf m, Gr
 // 1. The segments with a flagged block: entry {first block, touched mask |
 // to-clear mask << 4} (bit k: block first + k).  One thread per block; those
 // with bx % 4 == 0 read their segment's flags.  One list append (atomic) per
-// workgroup.  *count is zero on entry.
+// workgroup.  *count is zero on entry.  single: every block belongs to scene
+// 0, whose row length is then read once into scalar registers (no per-block
+// scene and config loads before the flags).
 __global__ __launch_bounds__(256) void k_grid_list(const DevCfg* __restrict__ cfgs,
                                                    const uint16_t* __restrict__ block_scene,
                                                    const uint8_t* __restrict__ mflags, uint8_t* __restrict__ vflags,
                                                    int clear_v, int nblocks, uint2* __restrict__ list,
-                                                   uint32_t* __restrict__ count) {
+                                                   uint32_t* __restrict__ count, int single) {
     __shared__ uint32_t wcount[4];
     __shared__ uint32_t wbase;
     const int blk = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t fm = 0u, fv = 0u;
+    int foff = 0, fbx = 1;
+    if (single) {
+        foff = __builtin_amdgcn_readfirstlane(cfgs[0].foff);
+        fbx = __builtin_amdgcn_readfirstlane(cfgs[0].fbx);
+    }
     if (blk < nblocks) {
-        const DevCfg& c = cfgs[block_scene[blk]];
-        const int bx = (blk - c.foff) % c.fbx;
+        if (!single) {
+            const DevCfg& c = cfgs[block_scene[blk]];
+            foff = c.foff;
+            fbx = c.fbx;
+        }
+        const int bx = (blk - foff) % fbx;
         if ((bx & 3) == 0) {
-            const int nk = min(4, c.fbx - bx);
+            const int nk = min(4, fbx - bx);
             for (int k = 0; k < nk; ++k) {
                 if (mflags[blk + k]) fm |= 1u << k;
                 if (clear_v && vflags[blk + k]) {
@@ -525,10 +536,10 @@ void launch_grid_update(const DevCfg* cfgs, const uint16_t* block_scene, int nsc
                         uint32_t* next_count, hipStream_t s, const CorrGridOut* corr) {
     if (nblocks <= 0) return;
     if (!next_count) hipLaunchKernelGGL(k_set_u32_grid, dim3(1), dim3(1), 0, s, count, 0u);
-    hipLaunchKernelGGL(k_grid_list, dim3((nblocks + 255) / 256), dim3(256), 0, s, cfgs, block_scene, m.flags,
-                       v_clear.flags, clear_v ? 1 : 0, nblocks, list, count);
-    const int waves = std::min((nblocks + 3) / 4, 8192);
     const int single = nscenes == 1 ? 1 : 0;
+    hipLaunchKernelGGL(k_grid_list, dim3((nblocks + 255) / 256), dim3(256), 0, s, cfgs, block_scene, m.flags,
+                       v_clear.flags, clear_v ? 1 : 0, nblocks, list, count, single);
+    const int waves = std::min((nblocks + 3) / 4, 8192);
     if (corr)
         hipLaunchKernelGGL(k_grid_update_corr, dim3((waves + 3) / 4), dim3(256), 0, s, cfgs, block_scene, m, v_clear,
                            sdf, list, count, next_count, single, nblocks, corr->vcopy, corr->active, corr->acap,
