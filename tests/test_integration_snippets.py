@@ -1,4 +1,4 @@
-"""The C and C++ bindings shown in INTEGRATION.md compile against include/.
+"""The C and C++ bindings shown in the integration guide compile against include/.
 
 Every fenced ```c block is compiled with gcc and every ```cpp block with g++
 (-fsyntax-only), so the documented replacement of the reference's
@@ -41,7 +41,7 @@ def test_snippets_compile(lang, compiler, flags, tmp_path):
         r = subprocess.run([compiler, *flags, "-fsyntax-only", "-Wall", "-Werror", "-Wno-unused-variable",
                             "-Wno-unused-but-set-variable", f"-I{ROOT / 'include'}", str(src)],
                            capture_output=True, text=True)
-        assert r.returncode == 0, f"INTEGRATION.md {lang} block {i} does not compile:\n{code}\n{r.stderr}"
+        assert r.returncode == 0, f"integration guide {lang} block {i} does not compile:\n{code}\n{r.stderr}"
 
 
 def test_parse_file_signature_matches_doc():
