@@ -243,8 +243,8 @@ def main() -> int:
         return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: every box of this build has one GPU, and the engine is "
-              "single-GPU (multi-GPU slab decomposition is a design, not part of this build); use --gpus 1",
+        print("bench.py runs on one GPU only: the engine is single-GPU (a multi-GPU slab decomposition "
+              "is designed but not implemented); use --gpus 1",
               file=sys.stderr)
         return 2
 

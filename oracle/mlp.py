@@ -1,6 +1,6 @@
 """CPU oracle of the learned grid-velocity corrector (test infrastructure only:
-tests and bench's CPU baseline use it as the checker; the product never calls
-it).
+tests and bench's CPU baseline compare the product against it; the product
+never calls it).
 
 The corrector has no counterpart in the reference; this module restates the
 network, its loss and gradient, and the per-node feature layout from their

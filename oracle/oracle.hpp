@@ -1,7 +1,7 @@
 // oracle.hpp -- CPU restatement of the reference MLS-MPM substep.
 //
-// TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP engine is
-// compared against.  Only tests/, __graft_entry__.smoke() and bench.py's
+// TEST INFRASTRUCTURE ONLY.  This file is the reference the HIP engine's
+// results are compared against.  Only tests/, __graft_entry__.smoke() and bench.py's
 // cpu_baseline leg may load the library built from it; the product path
 // (water-sim-cuda_amd/, include/) never links, calls or falls back to it.
 //

@@ -1,7 +1,8 @@
 """ctypes wrapper of liboracle.so -- the CPU restatement of the reference step.
 
 TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py as the checker.  The engine never imports it.
+cpu_baseline leg of bench.py, to compare the engine against.  The engine
+never imports it.
 """
 from __future__ import annotations
 

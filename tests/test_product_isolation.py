@@ -1,6 +1,6 @@
 """The product does not run the oracle.
 
-oracle/ is the checker: only tests/, __graft_entry__.smoke() and bench.py's
+oracle/ is test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
 CPU-baseline leg may import or load it.  These tests fail if any module of the
 package (Python or native source) imports or loads anything under oracle/, or
 if bench.py touches the oracle outside its cpu_baseline function.
