@@ -42,6 +42,39 @@ constexpr int SC_TILE = SC_THREADS * SC_ITEMS;  // 2048 values
 
 __device__ __forceinline__ uint64_t lanemask_lt(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
 
+// A value from another lane over the DPP network (an operand modifier of a
+// VALU instruction: no LDS round trip, unlike a shuffle).  Lanes whose source
+// lies outside the pattern, or whose row is not in ROW_MASK, read 0.  CTRL:
+// row_shr:n = 0x110 + n (within rows of 16 lanes), row_bcast:15 = 0x142 and
+// row_bcast:31 = 0x143 (the last lane of the previous row, of the lower half).
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ uint32_t dpp_read(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xF, false);
+}
+
+// Inclusive sum and maximum over a full wave64 (every lane active): shifts by
+// 1, 2, 4 and 8 within each row, then the rows' totals carried into the rows
+// after them.  Values are at least 0, so a lane reading 0 leaves the maximum
+// unchanged.
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t x) {
+    x += dpp_read<0x111>(x);
+    x += dpp_read<0x112>(x);
+    x += dpp_read<0x114>(x);
+    x += dpp_read<0x118>(x);
+    x += dpp_read<0x142, 0xA>(x);
+    x += dpp_read<0x143, 0xC>(x);
+    return x;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t x) {
+    x = max(x, dpp_read<0x111>(x));
+    x = max(x, dpp_read<0x112>(x));
+    x = max(x, dpp_read<0x114>(x));
+    x = max(x, dpp_read<0x118>(x));
+    x = max(x, dpp_read<0x142, 0xA>(x));
+    x = max(x, dpp_read<0x143, 0xC>(x));
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
 // Lanes of the wave whose `valid` is set and whose 8-bit digit equals ours.
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
     uint64_t m = __ballot(valid);
@@ -557,13 +590,8 @@ __global__ __launch_bounds__(RO_THREADS) void k_reorder(ChunkDesc* __restrict__ 
         __syncthreads();
         if (tid < 64) {  // wave 0: cells 2 lane and 2 lane + 1
             const uint32_t a = cnt[2 * lane], b = cnt[2 * lane + 1];
-            uint32_t x = a + b, m = max(a, b);
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t y = __shfl_up(x, o);
-                if (lane >= o) x += y;
-                m = max(m, (uint32_t)__shfl_xor((int)m, o));
-            }
+            const uint32_t x = wave_inclusive_sum(a + b);
+            const uint32_t m = wave_max(max(a, b));
             cfirst[2 * lane] = x - a - b;
             cfirst[2 * lane + 1] = x - b;
             if (lane == 0) cmax = m;
